@@ -1,0 +1,690 @@
+// C ABI of the (3,4)-nucleus path: a device context that owns every device
+// buffer (no global device symbols, so contexts are reentrant), and the stage
+// orchestration: orientation -> triangles -> incidence -> support ->
+// [K4 list + triangle->K4 incidence] -> peel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nuc34.h"
+#include "../host/graph_io.hpp"
+#include "cliques.hpp"
+#include "kernels.hpp"
+#include "scan.hpp"
+#include "segsort.hpp"
+
+using namespace nuc34;
+
+struct nuc34_graph {
+  LoadedGraph lg;
+};
+
+struct nuc34_ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Allocator alloc;
+  std::string err;
+  // oriented graph
+  bool has_graph = false;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  DevBuf O, E, src;
+  // triangles
+  bool has_tris = false;
+  uint64_t T = 0;
+  DevBuf cntA, cntB, cntC, offA, triA, triBC;
+  // incidence + support
+  bool has_sup = false;
+  uint64_t Q = 0;
+  DevBuf incO, third, tri, sup, own, bigq, small;  // small: bigcount(u32) + total(u64)
+  // direct K4 list
+  bool has_k4 = false;
+  DevBuf k4off, qud, koff, incK;
+  // peel result
+  bool has_k = false;
+  DevBuf kv;
+  nuc34_stats st{};
+};
+
+namespace {
+
+int fail(nuc34_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename F>
+int guard(nuc34_ctx* c, F&& f) {
+  try {
+    NUC34_HIP(hipSetDevice(c->device));
+    return f();
+  } catch (const OutOfMemory& e) {
+    return fail(c, NUC34_E_OOM, e.what());
+  } catch (const Overflow& e) {
+    return fail(c, NUC34_E_OVERFLOW, e.what());
+  } catch (const HipError& e) {
+    return fail(c, NUC34_E_HIP, e.what());
+  } catch (const std::invalid_argument& e) {
+    return fail(c, NUC34_E_INVALID, e.what());
+  } catch (const std::exception& e) {
+    return fail(c, NUC34_E_INVALID, e.what());
+  }
+}
+
+// Milliseconds between two events recorded around `f` on the context stream.
+template <typename F>
+double timed(nuc34_ctx* c, F&& f) {
+  NUC34_HIP(hipEventRecord(c->ev0, c->s));
+  f();
+  NUC34_HIP(hipEventRecord(c->ev1, c->s));
+  NUC34_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  NUC34_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  return ms;
+}
+
+template <typename T>
+T read_scalar(nuc34_ctx* c, const void* dev) {
+  T v{};
+  NUC34_HIP(hipMemcpyAsync(&v, dev, sizeof(T), hipMemcpyDeviceToHost, c->s));
+  NUC34_HIP(hipStreamSynchronize(c->s));
+  return v;
+}
+
+void zero(nuc34_ctx* c, DevBuf& b) {
+  if (b) NUC34_HIP(hipMemsetAsync(b.as<void>(), 0, b.bytes(), c->s));
+}
+
+void reset_stages(nuc34_ctx* c, int from) {
+  // from: 0 graph, 1 triangles, 2 support, 3 k4, 4 k
+  if (from <= 4) {
+    c->kv.release();
+    c->has_k = false;
+  }
+  if (from <= 3) {
+    c->k4off.release();
+    c->qud.release();
+    c->koff.release();
+    c->incK.release();
+    c->has_k4 = false;
+  }
+  if (from <= 2) {
+    for (DevBuf* b : {&c->incO, &c->third, &c->tri, &c->sup, &c->own, &c->bigq, &c->small}) b->release();
+    c->has_sup = false;
+    c->Q = 0;
+  }
+  if (from <= 1) {
+    for (DevBuf* b : {&c->cntA, &c->cntB, &c->cntC, &c->offA, &c->triA, &c->triBC}) b->release();
+    c->has_tris = false;
+    c->T = 0;
+  }
+  if (from <= 0) {
+    c->O.release();
+    c->E.release();
+    c->src.release();
+    c->has_graph = false;
+  }
+}
+
+// Edge ranges of `parts` vertex segments (make_split_parts on the oriented CSR).
+std::vector<std::pair<uint64_t, uint64_t>> edge_parts(nuc34_ctx* c, uint32_t parts) {
+  std::vector<std::pair<uint64_t, uint64_t>> r;
+  if (parts <= 1) {
+    r.emplace_back(0, c->m);
+    return r;
+  }
+  Csr h;
+  h.n = c->n;
+  h.O.resize(static_cast<size_t>(c->n) + 1);
+  NUC34_HIP(hipMemcpyAsync(h.O.data(), c->O.as<void>(), h.O.size() * 4, hipMemcpyDeviceToHost, c->s));
+  NUC34_HIP(hipStreamSynchronize(c->s));
+  const auto sp = split_parts(h, parts);
+  for (size_t i = 0; i < sp.size(); ++i) {
+    const uint64_t vb = sp[i];
+    const uint64_t ve = i + 1 < sp.size() ? sp[i + 1] : c->n;
+    if (ve <= vb) continue;
+    r.emplace_back(h.O[vb], h.O[ve]);
+  }
+  return r;
+}
+
+void run_triangles(nuc34_ctx* c, uint32_t parts) {
+  const uint64_t m = c->m;
+  const auto ranges = edge_parts(c, parts);
+  c->st.ms_triangles = timed(c, [&] {
+    c->cntA = c->alloc.get(m * 4 + 4, "triangle a-role counts");
+    c->cntB = c->alloc.get(m * 4 + 4, "triangle b-role counts");
+    c->cntC = c->alloc.get(m * 4 + 4, "triangle c-role counts");
+    zero(c, c->cntA);
+    zero(c, c->cntB);
+    zero(c, c->cntC);
+    for (const auto& r : ranges)
+      launch_tri_count(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second,
+                       c->cntA.as<uint32_t>(), c->cntB.as<uint32_t>(), c->cntC.as<uint32_t>(), c->s);
+    c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(c->cntA.as<uint32_t>(), c->offA.as<uint64_t>(), m,
+                                         scratch.as<uint64_t>(), c->s);
+      c->T = read_scalar<uint64_t>(c, c->offA.as<uint64_t>() + m);
+    }
+    if (c->T >= 0xFFFFFFFFull)
+      throw Overflow("triangle count " + std::to_string(c->T) + " exceeds the 32-bit triangle id range");
+    c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
+    c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
+    DevBuf cur = c->alloc.get(m * 4 + 4, "triangle cursors");
+    zero(c, cur);
+    for (const auto& r : ranges)
+      launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second,
+                      c->offA.as<uint64_t>(), cur.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
+                      c->s);
+    segmented_sort_u64(c->triBC.as<uint64_t>(), c->offA.as<uint64_t>(), m, c->alloc, c->s);
+  });
+  c->has_tris = true;
+  c->st.triangles = c->T;
+}
+
+void run_support(nuc34_ctx* c) {
+  const uint64_t m = c->m, T = c->T;
+  c->st.ms_incidence = timed(c, [&] {
+    DevBuf len = c->alloc.get(m * 4 + 4, "incidence lengths");
+    launch_inc_len(c->cntA.as<uint32_t>(), c->cntB.as<uint32_t>(), c->cntC.as<uint32_t>(), m, len.as<uint32_t>(),
+                   c->s);
+    c->incO = c->alloc.get((m + 1) * 8, "incidence offsets");
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(len.as<uint32_t>(), c->incO.as<uint64_t>(), m, scratch.as<uint64_t>(),
+                                         c->s);
+    }
+    len.release();
+    DevBuf inc = c->alloc.get(3 * T * 8 + 8, "incidence entries");
+    {
+      DevBuf cur = c->alloc.get(m * 4 + 4, "incidence cursors");
+      zero(c, cur);
+      launch_inc_fill(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->E.as<uint32_t>(),
+                      c->src.as<uint32_t>(), c->incO.as<uint64_t>(), cur.as<uint32_t>(), inc.as<uint64_t>(), c->s);
+    }
+    segmented_sort_u64(inc.as<uint64_t>(), c->incO.as<uint64_t>(), m, c->alloc, c->s);
+    c->third = c->alloc.get(3 * T * 4 + 4, "incidence third vertices");
+    c->tri = c->alloc.get(3 * T * 4 + 4, "incidence triangles");
+    launch_inc_split(inc.as<uint64_t>(), 3 * T, c->third.as<uint32_t>(), c->tri.as<uint32_t>(), c->s);
+  });
+  c->cntA.release();
+  c->cntB.release();
+  c->cntC.release();
+  c->st.incidence_entries = 3 * T;
+  c->st.ms_support = timed(c, [&] {
+    c->sup = c->alloc.get(T * 4 + 4, "support");
+    c->own = c->alloc.get(T * 4 + 4, "owned K4 counts");
+    c->bigq = c->alloc.get(T * 4 + 4, "wave queue");
+    c->small = c->alloc.get(64, "counters");
+    zero(c, c->small);
+    TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
+              c->tri.as<uint32_t>()};
+    launch_support(v, c->sup.as<int32_t>(), c->own.as<uint32_t>(),
+                   reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->bigq.as<uint32_t>(),
+                   c->small.as<uint32_t>(), c->s);
+    const uint64_t total = read_scalar<uint64_t>(c, c->small.as<char>() + 8);
+    if (total % 4) throw std::logic_error("support sum is not a multiple of 4");
+    c->Q = total / 4;
+  });
+  c->has_sup = true;
+  c->st.four_cliques = c->Q;
+}
+
+void run_k4_list(nuc34_ctx* c) {
+  const uint64_t T = c->T, Q = c->Q;
+  if (Q >= 0xFFFFFFFFull)
+    throw Overflow("four-clique count " + std::to_string(Q) + " exceeds the 32-bit K4 id range of the direct variant");
+  c->st.ms_k4_list = timed(c, [&] {
+    c->k4off = c->alloc.get((T + 1) * 8, "K4 offsets");
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), c->k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
+                                         c->s);
+    }
+    const uint64_t q2 = read_scalar<uint64_t>(c, c->k4off.as<uint64_t>() + T);
+    if (q2 != Q) throw std::logic_error("owned K4 count disagrees with the support sum");
+    c->qud = c->alloc.get(Q * 16 + 16, "K4 list");
+    TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
+              c->tri.as<uint32_t>()};
+    launch_k4_emit(v, c->own.as<uint32_t>(), c->k4off.as<uint64_t>(), c->qud.as<uint32_t>(), c->bigq.as<uint32_t>(),
+                   c->small.as<uint32_t>(), c->s);
+    c->koff = c->alloc.get((T + 1) * 8, "triangle->K4 offsets");
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
+      exclusive_scan<int32_t, uint64_t>(c->sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
+                                        c->s);
+    }
+    c->incK = c->alloc.get(4 * Q * 4 + 4, "triangle->K4 incidence");
+    DevBuf cur = c->alloc.get(T * 4 + 4, "K4 cursors");
+    zero(c, cur);
+    launch_k4_incidence(c->qud.as<uint32_t>(), Q, c->koff.as<uint64_t>(), cur.as<uint32_t>(), c->incK.as<uint32_t>(),
+                        c->s);
+  });
+  c->has_k4 = true;
+}
+
+void run_peel(nuc34_ctx* c, int variant) {
+  const uint64_t T = c->T;
+  c->kv = c->alloc.get(T * 4 + 4, "k values");
+  c->has_k = true;
+  c->st.rounds = 0;
+  c->st.k_max = 0;
+  c->st.unpeeled = 0;
+  if (!T) return;
+  if (!c->Q) {
+    zero(c, c->kv);
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    return;
+  }
+  const uint32_t T32 = static_cast<uint32_t>(T);
+  DevBuf dd = c->alloc.get(T * 4, "peel supports");
+  DevBuf rnd = c->alloc.get(T * 4, "round stamps");
+  DevBuf alive = c->alloc.get(T * 4, "alive list");
+  DevBuf alive2 = c->alloc.get(T * 4, "alive list");
+  DevBuf front = c->alloc.get(T * 4, "frontier");
+  DevBuf next = c->alloc.get(T * 4, "frontier");
+  DevBuf pcb = c->alloc.get(sizeof(PeelCounters), "peel counters");
+  PeelCounters* pc = pcb.as<PeelCounters>();
+  TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
+            c->tri.as<uint32_t>()};
+  int32_t level = 0;
+  uint32_t round = 0;
+  uint64_t peeled = 0;
+  c->st.ms_peel = timed(c, [&] {
+    launch_fill_i32(c->kv.as<int32_t>(), T, -1, c->s);
+    launch_fill_u32(rnd.as<uint32_t>(), T, 0xFFFFFFFFu, c->s);
+    NUC34_HIP(hipMemcpyAsync(dd.as<void>(), c->sup.as<void>(), T * 4, hipMemcpyDeviceToDevice, c->s));
+    launch_iota(alive.as<uint32_t>(), T, c->s);
+    NUC34_HIP(hipMemsetAsync(pc, 0, sizeof(PeelCounters), c->s));
+    uint32_t nalive = T32, fcount = 0;
+    uint64_t last_compact = T;
+    while (peeled < T) {
+      if (fcount == 0) {
+        NUC34_HIP(hipMemsetAsync(&pc->minval, 0xFF, 4, c->s));
+        launch_alive_min(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), dd.as<int32_t>(), pc, c->s);
+        const uint32_t mn = read_scalar<uint32_t>(c, &pc->minval);
+        if (mn == 0xFFFFFFFFu) throw std::logic_error("peel: no alive triangle left but peeled < T");
+        level = std::max<int32_t>(level, static_cast<int32_t>(mn));
+        NUC34_HIP(hipMemsetAsync(&pc->front, 0, 4, c->s));
+        launch_collect(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), dd.as<int32_t>(), rnd.as<uint32_t>(), level,
+                       round, front.as<uint32_t>(), pc, c->s);
+        fcount = read_scalar<uint32_t>(c, &pc->front);
+        if (!fcount) throw std::logic_error("peel: empty frontier at a new level");
+      } else {
+        launch_mark(front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(), level, round, c->s);
+      }
+      NUC34_HIP(hipMemsetAsync(&pc->next, 0, 8, c->s));  // next, big
+      if (variant == NUC34_DIRECT) {
+        launch_debit_direct(c->koff.as<uint64_t>(), c->incK.as<uint32_t>(), c->qud.as<uint32_t>(),
+                            front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(), dd.as<int32_t>(), level,
+                            round, next.as<uint32_t>(), c->s);
+      } else {
+        launch_debit_ondemand(v, front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(),
+                              dd.as<int32_t>(), level, round, next.as<uint32_t>(), c->bigq.as<uint32_t>(), c->s);
+      }
+      const uint32_t ncount = read_scalar<uint32_t>(c, &pc->next);
+      peeled += fcount;
+      ++round;
+      std::swap(front, next);
+      NUC34_HIP(hipMemcpyAsync(&pc->front, &pc->next, 4, hipMemcpyDeviceToDevice, c->s));
+      fcount = ncount;
+      if (peeled < T && (T - peeled) * 2 <= last_compact) {
+        NUC34_HIP(hipMemsetAsync(&pc->alive, 0, 4, c->s));
+        launch_compact_alive(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), alive2.as<uint32_t>(), pc, c->s);
+        nalive = read_scalar<uint32_t>(c, &pc->alive);
+        std::swap(alive, alive2);
+        last_compact = nalive;
+      }
+    }
+  });
+  c->st.rounds = round;
+  c->st.k_max = level;
+  c->st.unpeeled = T - peeled;
+}
+
+// Host-side checks that keep every device index in bounds: monotone offsets,
+// ids < n, strictly increasing lists without self-loops, and for oriented
+// input an acyclic orientation (Kahn's algorithm).
+std::string validate_csr(const uint32_t* O, const uint32_t* E, uint32_t n, bool oriented) {
+  for (uint32_t u = 0; u < n; ++u) {
+    if (O[u] > O[u + 1]) return "offsets not monotone at vertex " + std::to_string(u);
+    for (uint32_t i = O[u]; i < O[u + 1]; ++i) {
+      if (E[i] >= n) return "vertex id " + std::to_string(E[i]) + " out of range";
+      if (E[i] == u) return "self-loop at vertex " + std::to_string(u);
+      if (i > O[u] && E[i - 1] >= E[i]) return "adjacency of vertex " + std::to_string(u) + " not strictly increasing";
+    }
+  }
+  if (oriented) {
+    std::vector<uint32_t> indeg(n, 0), queue;
+    for (uint64_t i = 0; i < O[n]; ++i) indeg[E[i]]++;
+    queue.reserve(n);
+    for (uint32_t u = 0; u < n; ++u)
+      if (!indeg[u]) queue.push_back(u);
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const uint32_t u = queue[h];
+      for (uint32_t i = O[u]; i < O[u + 1]; ++i)
+        if (--indeg[E[i]] == 0) queue.push_back(E[i]);
+    }
+    if (queue.size() != n) return "oriented graph has a directed cycle";
+  }
+  return std::string();
+}
+
+uint64_t direct_bytes(const nuc34_ctx* c) {
+  // K4 list, triangle->K4 incidence, two offset arrays, scan scratch
+  return c->Q * 16 + c->Q * 16 + (c->T + 1) * 16 + (c->T / 1024 + 64) * 8;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc34_graph_load(const char* path, nuc34_graph** out, char* err, size_t errlen) {
+  try {
+    auto* g = new nuc34_graph;
+    try {
+      g->lg = load_dataset(path);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+    return NUC34_OK;
+  } catch (const std::exception& e) {
+    if (err && errlen) {
+      std::strncpy(err, e.what(), errlen - 1);
+      err[errlen - 1] = 0;
+    }
+    return NUC34_E_IO;
+  }
+}
+
+int nuc34_graph_from_pairs(const uint32_t* src, const uint32_t* dst, uint64_t count, nuc34_graph** out) {
+  try {
+    auto* g = new nuc34_graph;
+    g->lg.g = csr_from_pairs(src, dst, count);
+    g->lg.lines = count;
+    *out = g;
+    return NUC34_OK;
+  } catch (const std::exception&) {
+    return NUC34_E_INVALID;
+  }
+}
+
+int nuc34_graph_info(const nuc34_graph* g, uint32_t* n, uint64_t* m, int* oriented) {
+  if (!g) return NUC34_E_INVALID;
+  if (n) *n = g->lg.g.n;
+  if (m) *m = g->lg.g.m();
+  if (oriented) *oriented = g->lg.oriented ? 1 : 0;
+  return NUC34_OK;
+}
+
+int nuc34_graph_copy(const nuc34_graph* g, uint32_t* O, uint32_t* E) {
+  if (!g) return NUC34_E_INVALID;
+  std::memcpy(O, g->lg.g.O.data(), g->lg.g.O.size() * 4);
+  if (!g->lg.g.E.empty()) std::memcpy(E, g->lg.g.E.data(), g->lg.g.E.size() * 4);
+  return NUC34_OK;
+}
+
+int nuc34_graph_orient_host(nuc34_graph* g) {
+  if (!g) return NUC34_E_INVALID;
+  if (!g->lg.oriented) {
+    g->lg.g = orient_host(g->lg.g);
+    g->lg.oriented = true;
+  }
+  return NUC34_OK;
+}
+
+void nuc34_graph_free(nuc34_graph* g) { delete g; }
+
+int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
+  auto* c = new nuc34_ctx;
+  c->device = device;
+  c->alloc.budget = hbm_budget;
+  const int rc = guard(c, [&]() -> int {
+    NUC34_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    NUC34_HIP(hipEventCreate(&c->ev0));
+    NUC34_HIP(hipEventCreate(&c->ev1));
+    return NUC34_OK;
+  });
+  if (rc != NUC34_OK) {
+    delete c;
+    *out = nullptr;
+    return rc;
+  }
+  *out = c;
+  return NUC34_OK;
+}
+
+void nuc34_ctx_destroy(nuc34_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  reset_stages(c, 0);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+const char* nuc34_last_error(const nuc34_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int oriented) {
+  if (!c || !O || (m && !E)) return fail(c, NUC34_E_INVALID, "null argument");
+  if (O[0] != 0 || O[n] != m) return fail(c, NUC34_E_INVALID, "offsets do not span the edge array");
+  if (m >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "more than 2^32-1 edges");
+  {
+    const std::string bad = validate_csr(O, E, n, oriented != 0);
+    if (!bad.empty()) return fail(c, NUC34_E_INVALID, bad);
+  }
+  return guard(c, [&]() -> int {
+    reset_stages(c, 0);
+    c->st = nuc34_stats{};
+    c->n = n;
+    DevBuf Ou = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "input offsets");
+    DevBuf Eu = c->alloc.get(m * 4 + 4, "input edges");
+    NUC34_HIP(hipMemcpyAsync(Ou.as<void>(), O, (static_cast<uint64_t>(n) + 1) * 4, hipMemcpyHostToDevice, c->s));
+    if (m) NUC34_HIP(hipMemcpyAsync(Eu.as<void>(), E, m * 4, hipMemcpyHostToDevice, c->s));
+    if (oriented) {
+      c->m = m;
+      c->O = std::move(Ou);
+      c->E = std::move(Eu);
+      c->src = c->alloc.get(m * 4 + 4, "edge sources");
+      launch_fill_src(c->O.as<uint32_t>(), n, c->src.as<uint32_t>(), c->s);
+      NUC34_HIP(hipStreamSynchronize(c->s));
+      c->st.ms_orient = 0;
+    } else {
+      if (m % 2) return fail(c, NUC34_E_INVALID, "undirected CSR must hold both directions of every edge");
+      c->st.ms_orient = timed(c, [&] {
+        DevBuf deg = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "out-degrees");
+        launch_orient_count(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, deg.as<uint32_t>(), c->s);
+        c->O = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "oriented offsets");
+        DevBuf scratch = c->alloc.get(scan_scratch_elems(n) * 4, "scan scratch");
+        exclusive_scan<uint32_t, uint32_t>(deg.as<uint32_t>(), c->O.as<uint32_t>(), n, scratch.as<uint32_t>(),
+                                           c->s);
+        const uint32_t mo = read_scalar<uint32_t>(c, c->O.as<uint32_t>() + n);
+        if (mo != m / 2) throw std::invalid_argument("undirected CSR is not symmetric");
+        c->m = mo;
+        c->E = c->alloc.get(c->m * 4 + 4, "oriented edges");
+        c->src = c->alloc.get(c->m * 4 + 4, "edge sources");
+        launch_orient_fill(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, c->O.as<uint32_t>(), c->E.as<uint32_t>(),
+                           c->src.as<uint32_t>(), c->s);
+      });
+    }
+    c->has_graph = true;
+    c->st.n = n;
+    c->st.m = c->m;
+    return NUC34_OK;
+  });
+}
+
+int nuc34_graph_dims(const nuc34_ctx* c, uint32_t* n, uint64_t* m) {
+  if (!c || !c->has_graph) return NUC34_E_STATE;
+  if (n) *n = c->n;
+  if (m) *m = c->m;
+  return NUC34_OK;
+}
+
+int nuc34_copy_oriented(nuc34_ctx* c, uint32_t* O, uint32_t* E) {
+  if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
+  return guard(c, [&]() -> int {
+    NUC34_HIP(hipMemcpyAsync(O, c->O.as<void>(), (static_cast<uint64_t>(c->n) + 1) * 4, hipMemcpyDeviceToHost, c->s));
+    if (c->m) NUC34_HIP(hipMemcpyAsync(E, c->E.as<void>(), c->m * 4, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    return NUC34_OK;
+  });
+}
+
+int nuc34_triangles(nuc34_ctx* c, uint32_t parts, uint64_t* T) {
+  if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
+  return guard(c, [&]() -> int {
+    if (!c->has_tris) {
+      reset_stages(c, 1);
+      run_triangles(c, parts);
+    }
+    if (T) *T = c->T;
+    return NUC34_OK;
+  });
+}
+
+int nuc34_copy_triangles(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t* abc) {
+  if (!c || !c->has_tris) return fail(c, NUC34_E_STATE, "triangles not computed");
+  if (first > c->T || count > c->T - first) return fail(c, NUC34_E_INVALID, "triangle range out of bounds");
+  return guard(c, [&]() -> int {
+    std::vector<uint32_t> a(count);
+    std::vector<uint64_t> bc(count);
+    if (count) {
+      NUC34_HIP(hipMemcpyAsync(a.data(), c->triA.as<uint32_t>() + first, count * 4, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipMemcpyAsync(bc.data(), c->triBC.as<uint64_t>() + first, count * 8, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    }
+    for (uint64_t i = 0; i < count; ++i) {
+      abc[3 * i] = a[i];
+      abc[3 * i + 1] = static_cast<uint32_t>(bc[i] >> 32);
+      abc[3 * i + 2] = static_cast<uint32_t>(bc[i]);
+    }
+    return NUC34_OK;
+  });
+}
+
+int nuc34_support(nuc34_ctx* c, uint64_t* Q) {
+  if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
+  return guard(c, [&]() -> int {
+    if (!c->has_tris) run_triangles(c, 1);
+    if (!c->has_sup) {
+      reset_stages(c, 2);
+      if (c->T) run_support(c);
+      else c->has_sup = true;
+    }
+    if (Q) *Q = c->Q;
+    return NUC34_OK;
+  });
+}
+
+int nuc34_copy_support(nuc34_ctx* c, int32_t* support) {
+  if (!c || !c->has_sup) return fail(c, NUC34_E_STATE, "support not computed");
+  return guard(c, [&]() -> int {
+    if (c->T) {
+      NUC34_HIP(hipMemcpyAsync(support, c->sup.as<void>(), c->T * 4, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    }
+    return NUC34_OK;
+  });
+}
+
+int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stats) {
+  if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
+  if (variant != NUC34_DIRECT && variant != NUC34_ONDEMAND && variant != NUC34_AUTO)
+    return fail(c, NUC34_E_INVALID, "unknown variant");
+  return guard(c, [&]() -> int {
+    if (!c->has_tris) {
+      reset_stages(c, 1);
+      run_triangles(c, parts ? parts : 1);
+    }
+    if (!c->has_sup) {
+      reset_stages(c, 2);
+      if (c->T) run_support(c);
+      else c->has_sup = true;
+    }
+    int v = variant;
+    if (v == NUC34_AUTO) {
+      size_t free_b = 0, total_b = 0;
+      NUC34_HIP(hipMemGetInfo(&free_b, &total_b));
+      uint64_t room = static_cast<uint64_t>(0.8 * static_cast<double>(free_b));
+      if (c->alloc.budget) room = std::min<uint64_t>(room, c->alloc.budget > c->alloc.used ? c->alloc.budget - c->alloc.used : 0);
+      v = (c->Q < 0xFFFFFFFFull && direct_bytes(c) + c->T * 28 <= room) ? NUC34_DIRECT : NUC34_ONDEMAND;
+    }
+    reset_stages(c, 4);
+    c->st.ms_k4_list = 0;
+    if (v == NUC34_DIRECT && c->Q && !c->has_k4) run_k4_list(c);
+    run_peel(c, v);
+    c->st.variant = v;
+    c->st.peak_bytes = c->alloc.peak;
+    c->st.ms_total = c->st.ms_triangles + c->st.ms_incidence + c->st.ms_support + c->st.ms_k4_list + c->st.ms_peel;
+    if (stats) *stats = c->st;
+    return NUC34_OK;
+  });
+}
+
+int nuc34_copy_k(nuc34_ctx* c, int32_t* k) {
+  if (!c || !c->has_k) return fail(c, NUC34_E_STATE, "decomposition not run");
+  return guard(c, [&]() -> int {
+    if (c->T) {
+      NUC34_HIP(hipMemcpyAsync(k, c->kv.as<void>(), c->T * 4, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    }
+    return NUC34_OK;
+  });
+}
+
+int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t* quds) {
+  if (!c || !c->has_k4) return fail(c, NUC34_E_STATE, "K4 list not built (direct variant only)");
+  if (first > c->Q || count > c->Q - first) return fail(c, NUC34_E_INVALID, "K4 range out of bounds");
+  return guard(c, [&]() -> int {
+    if (count) {
+      NUC34_HIP(hipMemcpyAsync(quds, c->qud.as<uint32_t>() + 4 * first, count * 16, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    }
+    return NUC34_OK;
+  });
+}
+
+int nuc34_get_stats(const nuc34_ctx* c, nuc34_stats* stats) {
+  if (!c || !stats) return NUC34_E_INVALID;
+  *stats = c->st;
+  return NUC34_OK;
+}
+
+int nuc34_device_info(int device, char* buf, size_t len) {
+  hipDeviceProp_t p{};
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return NUC34_E_HIP;
+  int rt = 0, drv = 0;
+  (void)hipRuntimeGetVersion(&rt);
+  (void)hipDriverGetVersion(&drv);
+  std::string s;
+  s += "name=" + std::string(p.name) + "\n";
+  s += "arch=" + std::string(p.gcnArchName) + "\n";
+  s += "global_memory_bytes=" + std::to_string(p.totalGlobalMem) + "\n";
+  s += "compute_units=" + std::to_string(p.multiProcessorCount) + "\n";
+  s += "clock_khz=" + std::to_string(p.clockRate) + "\n";
+  s += "memory_clock_khz=" + std::to_string(p.memoryClockRate) + "\n";
+  s += "memory_bus_width=" + std::to_string(p.memoryBusWidth) + "\n";
+  s += "l2_cache_bytes=" + std::to_string(p.l2CacheSize) + "\n";
+  s += "lds_per_block_bytes=" + std::to_string(p.sharedMemPerBlock) + "\n";
+  s += "warp_size=" + std::to_string(p.warpSize) + "\n";
+  s += "pci_bus_id=" + std::to_string(p.pciBusID) + "\n";
+  s += "hip_runtime=" + std::to_string(rt) + "\n";
+  s += "hip_driver=" + std::to_string(drv) + "\n";
+  if (!buf || !len) return NUC34_E_INVALID;
+  std::strncpy(buf, s.c_str(), len - 1);
+  buf[len - 1] = 0;
+  return NUC34_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Shared definitions for the gfx950 kernels of the (3,4)-nucleus path:
+// wave64 helpers, HIP error handling and device buffers owned by a context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace nuc34 {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // 4 waves per workgroup
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const std::string& where)
+      : std::runtime_error(where + ": " + hipGetErrorString(c)), code(c) {}
+};
+
+struct Overflow : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct OutOfMemory : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define NUC34_HIP(expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) throw ::nuc34::HipError(e_, std::string(#expr) + " at " __FILE__); \
+  } while (0)
+
+#define NUC34_LAUNCH_CHECK() NUC34_HIP(hipGetLastError())
+
+inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 1u << 20) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// ---- device-side helpers -------------------------------------------------
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of `mask` below this lane (mbcnt on the 64-bit ballot).
+__device__ __forceinline__ unsigned lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint64_t atomic_add_u64(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t atomic_sub_i32(int32_t* p, int32_t v) {
+  return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_min_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-aggregated append: every active lane with `pred` gets a unique slot of
+// *counter; one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, bool* got) {
+  const uint64_t mask = __ballot(pred);
+  *got = pred;
+  if (!mask) return 0;
+  const unsigned leader = static_cast<unsigned>(__builtin_ctzll(mask));
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomic_add_u32(counter, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, static_cast<int>(leader));
+  return base + lanes_below(mask);
+}
+
+// Wave-wide inclusive prefix sum of a 32-bit value.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(v, static_cast<unsigned>(off));
+    if (lane >= static_cast<unsigned>(off)) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// First index in sorted a[lo, hi) whose value is >= key.
+template <typename T>
+__device__ __forceinline__ uint64_t lower_bound_dev(const T* a, uint64_t lo, uint64_t hi, T key) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace nuc34
+
+namespace nuc34 {
+
+// Device allocations charged against a byte budget (0 = only the device limit).
+class Allocator;
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { swap(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      swap(o);
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release();
+  template <typename T>
+  T* as() const { return static_cast<T*>(p_); }
+  size_t bytes() const { return bytes_; }
+  explicit operator bool() const { return p_ != nullptr; }
+
+ private:
+  friend class Allocator;
+  void swap(DevBuf& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(bytes_, o.bytes_);
+    std::swap(owner_, o.owner_);
+  }
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+  Allocator* owner_ = nullptr;
+};
+
+class Allocator {
+ public:
+  uint64_t budget = 0;  // bytes; 0 = no limit beyond the device
+  uint64_t used = 0;
+  uint64_t peak = 0;
+  DevBuf get(size_t bytes, const char* what) {
+    DevBuf b;
+    if (!bytes) return b;
+    if (budget && used + bytes > budget)
+      throw OutOfMemory(std::string("HBM budget exceeded allocating ") + what + " (" +
+                        std::to_string(bytes) + " bytes; in use " + std::to_string(used) + " of " +
+                        std::to_string(budget) + ")");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+      (void)hipGetLastError();
+      throw OutOfMemory(std::string("device out of memory allocating ") + what + " (" +
+                        std::to_string(bytes) + " bytes)");
+    }
+    if (e != hipSuccess) throw HipError(e, std::string("hipMalloc ") + what);
+    b.p_ = p;
+    b.bytes_ = bytes;
+    b.owner_ = this;
+    used += bytes;
+    if (used > peak) peak = used;
+    return b;
+  }
+  void give_back(size_t bytes) { used -= bytes; }
+};
+
+inline void DevBuf::release() {
+  if (p_) {
+    (void)hipFree(p_);
+    if (owner_) owner_->give_back(bytes_);
+  }
+  p_ = nullptr;
+  bytes_ = 0;
+  owner_ = nullptr;
+}
+
+}  // namespace nuc34

@@ -1,0 +1,131 @@
+// (a) Degree orientation on the device: keep u->v iff (deg u, u) < (deg v, v)
+// with the undirected degree, list order preserved, so edge indices equal those
+// of the reference's host make_oriented (nucleus/graph/graph.hpp:176-218,
+// vertex_less at nucleus/graph/graph_api.hpp:124-127).
+// One wave per vertex; keep flags are ballots, positions come from mbcnt.
+#include "kernels.hpp"
+
+namespace nuc34 {
+
+namespace {
+
+__device__ __forceinline__ bool keep_edge(uint32_t u, uint32_t du, uint32_t v, uint32_t dv) {
+  return du < dv || (du == dv && u < v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_orient_count(const uint32_t* __restrict__ Ou,
+                                                         const uint32_t* __restrict__ Eu, uint32_t n,
+                                                         uint32_t* __restrict__ outdeg) {
+  const unsigned lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    const uint32_t b = Ou[u], e = Ou[u + 1], du = e - b;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < du; base += kWave) {
+      const uint32_t i = base + lane;
+      bool keep = false;
+      if (i < du) {
+        const uint32_t v = Eu[b + i];
+        keep = keep_edge(static_cast<uint32_t>(u), du, v, Ou[v + 1] - Ou[v]);
+      }
+      cnt += static_cast<uint32_t>(__popcll(__ballot(keep)));
+    }
+    if (lane == 0) outdeg[u] = cnt;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_orient_fill(const uint32_t* __restrict__ Ou,
+                                                        const uint32_t* __restrict__ Eu, uint32_t n,
+                                                        const uint32_t* __restrict__ O, uint32_t* __restrict__ E,
+                                                        uint32_t* __restrict__ src) {
+  const unsigned lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    const uint32_t b = Ou[u], e = Ou[u + 1], du = e - b;
+    uint32_t out = O[u];
+    for (uint32_t base = 0; base < du; base += kWave) {
+      const uint32_t i = base + lane;
+      bool keep = false;
+      uint32_t v = 0;
+      if (i < du) {
+        v = Eu[b + i];
+        keep = keep_edge(static_cast<uint32_t>(u), du, v, Ou[v + 1] - Ou[v]);
+      }
+      const uint64_t mask = __ballot(keep);
+      if (keep) {
+        const uint32_t pos = out + lanes_below(mask);
+        E[pos] = v;
+        src[pos] = static_cast<uint32_t>(u);
+      }
+      out += static_cast<uint32_t>(__popcll(mask));
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_src(const uint32_t* __restrict__ O, uint32_t n,
+                                                     uint32_t* __restrict__ src) {
+  const unsigned lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
+    for (uint32_t i = O[u] + lane; i < O[u + 1]; i += kWave) src[i] = static_cast<uint32_t>(u);
+  }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ a, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    a[i] = static_cast<uint32_t>(i);
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, uint64_t n, T v) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    a[i] = v;
+}
+
+}  // namespace
+
+void launch_orient_count(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, uint32_t* outdeg, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_orient_count, dim3(grid_for(n, kBlock / kWave, 1u << 16)), dim3(kBlock), 0, s, Ou, Eu, n,
+                     outdeg);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_orient_fill(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, const uint32_t* O, uint32_t* E,
+                        uint32_t* src, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_orient_fill, dim3(grid_for(n, kBlock / kWave, 1u << 16)), dim3(kBlock), 0, s, Ou, Eu, n, O,
+                     E, src);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_fill_src(const uint32_t* O, uint32_t n, uint32_t* src, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_src, dim3(grid_for(n, kBlock / kWave, 1u << 16)), dim3(kBlock), 0, s, O, n, src);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_iota(uint32_t* a, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, a, n);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_fill_i32(int32_t* a, uint64_t n, int32_t v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, a, n, v);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill<uint32_t>, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, a, n, v);
+  NUC34_LAUNCH_CHECK();
+}
+
+}  // namespace nuc34

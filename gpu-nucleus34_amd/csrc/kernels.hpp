@@ -1,0 +1,32 @@
+// Launch wrappers for the gfx950 kernels of the (3,4)-nucleus path. All device
+// pointers; every wrapper enqueues on stream `s` and checks the launch.
+#pragma once
+#include "common.hpp"
+
+namespace nuc34 {
+
+// (a) degree orientation of an undirected CSR (Ou[n+1], Eu[2m]).
+void launch_orient_count(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, uint32_t* outdeg, hipStream_t s);
+void launch_orient_fill(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, const uint32_t* O, uint32_t* E,
+                        uint32_t* src, hipStream_t s);
+void launch_fill_src(const uint32_t* O, uint32_t n, uint32_t* src, hipStream_t s);
+
+// (b) triangle enumeration over oriented edges [e_begin, e_end).
+void launch_tri_count(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
+                      uint64_t e_end, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s);
+void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
+                     uint64_t e_end, const uint64_t* offA, uint32_t* curA, uint32_t* triA, uint64_t* triBC,
+                     hipStream_t s);
+
+// (d) edge -> triangle incidence.
+void launch_inc_len(const uint32_t* cntA, const uint32_t* cntB, const uint32_t* cntC, uint64_t m, uint32_t* len,
+                    hipStream_t s);
+void launch_inc_fill(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const uint32_t* E,
+                     const uint32_t* src, const uint64_t* incO, uint32_t* cur, uint64_t* inc, hipStream_t s);
+void launch_inc_split(const uint64_t* inc, uint64_t n, uint32_t* third, uint32_t* tri, hipStream_t s);
+
+void launch_iota(uint32_t* a, uint64_t n, hipStream_t s);
+void launch_fill_i32(int32_t* a, uint64_t n, int32_t v, hipStream_t s);
+void launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t s);
+
+}  // namespace nuc34

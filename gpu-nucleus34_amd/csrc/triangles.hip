@@ -1,0 +1,154 @@
+// (b) Triangle enumeration by sorted-adjacency intersection.
+//
+// Reference: cu_cliques_tris + intersect_lists (nucleus/utils.cuh:259-343) and
+// cliques_tri_par (nucleus/cliques.hpp:98-226): for each oriented edge b=(u,v),
+// every w in N+(u) ∩ N+(v) gives the triangle {a = idx(u,w), b, c = idx(v,w)}.
+//
+// Here one wave takes a tile of 64 consecutive oriented edges and flattens the
+// work of the tile (sum of d+(v) over its edges) across its 64 lanes, so lane
+// utilisation does not depend on list lengths. The tile's owner vertices are
+// consecutive, so their out-lists are one contiguous range of E, staged in LDS
+// when it fits; each candidate w = E[c] is then found in N+(u) by binary search
+// in LDS (global memory otherwise).
+//   count pass: per-edge totals of the a-, b- and c-roles (a-role counts give the
+//               canonical offsets, all three the incidence lengths);
+//   emit pass:  each triangle lands in its a-edge segment (offsets from a scan of
+//               the a-role counts); a segmented sort by (b, c) then gives exactly
+//               the (a, b) order of the reference's sorted triangle list
+//               (tri_t::operator<, utils.cuh:93-102).
+#include "kernels.hpp"
+
+namespace nuc34 {
+
+namespace {
+
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kListCap = 1536;  // staged owner-list entries per wave
+
+template <typename P>
+__device__ __forceinline__ uint32_t search_u32(P list, uint32_t lo, uint32_t hi, uint32_t key) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (list[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
+                                                     const uint32_t* __restrict__ src, uint64_t e_begin,
+                                                     uint64_t e_end, uint32_t* __restrict__ cntA,
+                                                     uint32_t* __restrict__ cntB, uint32_t* __restrict__ cntC,
+                                                     const uint64_t* __restrict__ offA, uint32_t* __restrict__ curA,
+                                                     uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC) {
+  __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
+  __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_len[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_vb[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_cb[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_list[kWavesPerBlock][kListCap];
+  const unsigned w = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + w; e_begin + tile * kWave < e_end;
+       tile += nwaves) {
+    const uint64_t e0 = e_begin + tile * kWave;
+    const uint64_t e = e0 + lane;
+    const bool valid = e < e_end;
+    uint32_t work = 0, ob = 0, oe = 0, vb = 0;
+    if (valid) {
+      const uint32_t u = src[e];
+      const uint32_t v = E[e];
+      vb = O[v];
+      work = O[v + 1] - vb;
+      ob = O[u];
+      oe = O[u + 1];
+    }
+    const uint32_t inc = wave_inclusive_sum(work);
+    const uint32_t total = __shfl(inc, kWave - 1);
+    const uint64_t last_lane = (e_end - e0 < static_cast<uint64_t>(kWave) ? e_end - e0 : kWave) - 1;
+    const uint32_t lo = __shfl(ob, 0);
+    const uint32_t hi = __shfl(oe, static_cast<int>(last_lane));
+    const bool staged = (hi - lo) <= kListCap;
+    if (staged)
+      for (uint32_t i = lane; i < hi - lo; i += kWave) s_list[w][i] = E[lo + i];
+    s_pre[w][lane] = inc - work;
+    if (lane == kWave - 1) s_pre[w][kWave] = inc;
+    s_l0[w][lane] = staged ? ob - lo : ob;
+    s_len[w][lane] = oe - ob;
+    s_vb[w][lane] = vb;
+    s_cb[w][lane] = 0;
+    wave_sync();
+    for (uint32_t base = 0; base < total; base += kWave) {
+      const uint32_t item = base + lane;
+      if (item < total) {
+        uint32_t slo = 0, shi = kWave;
+        while (shi - slo > 1) {
+          const uint32_t mid = (slo + shi) >> 1;
+          if (s_pre[w][mid] <= item) slo = mid;
+          else shi = mid;
+        }
+        const uint32_t i = slo;
+        const uint32_t c = s_vb[w][i] + (item - s_pre[w][i]);
+        const uint32_t x = E[c];
+        const uint32_t l0 = s_l0[w][i];
+        const uint32_t l1 = l0 + s_len[w][i];
+        uint32_t pos;
+        bool found;
+        if (staged) {
+          pos = search_u32(s_list[w], l0, l1, x);
+          found = pos < l1 && s_list[w][pos] == x;
+          pos += lo;
+        } else {
+          pos = search_u32(E, l0, l1, x);
+          found = pos < l1 && E[pos] == x;
+        }
+        if (found) {
+          const uint32_t a = pos;
+          const uint32_t b = static_cast<uint32_t>(e0 + i);
+          if constexpr (!EMIT) {
+            atomic_add_u32(&cntA[a], 1u);
+            atomicAdd(&s_cb[w][i], 1u);
+            atomic_add_u32(&cntC[c], 1u);
+          } else {
+            const uint64_t p = offA[a] + atomic_add_u32(&curA[a], 1u);
+            triA[p] = a;
+            triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
+          }
+        }
+      }
+    }
+    wave_sync();
+    if constexpr (!EMIT) {
+      if (valid) cntB[e] = s_cb[w][lane];
+    }
+    wave_sync();
+  }
+}
+
+unsigned tri_grid(uint64_t edges) {
+  const uint64_t tiles = (edges + kWave - 1) / kWave;
+  return grid_for(tiles, kWavesPerBlock, 1u << 15);
+}
+
+}  // namespace
+
+void launch_tri_count(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
+                      uint64_t e_end, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s) {
+  if (e_end <= e_begin) return;
+  hipLaunchKernelGGL(k_tri_enum<false>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
+                     e_end, cntA, cntB, cntC, nullptr, nullptr, nullptr, nullptr);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
+                     uint64_t e_end, const uint64_t* offA, uint32_t* curA, uint32_t* triA, uint64_t* triBC,
+                     hipStream_t s) {
+  if (e_end <= e_begin) return;
+  hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
+                     e_end, nullptr, nullptr, nullptr, offA, curA, triA, triBC);
+  NUC34_LAUNCH_CHECK();
+}
+
+}  // namespace nuc34

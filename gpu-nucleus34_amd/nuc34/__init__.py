@@ -1,0 +1,257 @@
+"""Python host mirror of the (3,4)-nucleus C ABI (include/nuc34.h).
+
+Loads the in-tree build of libnuc34.so (gpu-nucleus34_amd/_build) and exposes
+the reference's operator surface: dataset loading with the reference loader
+semantics, and a device context that runs orientation, triangles, incidence,
+support and the direct or on-demand peel on the GPU. There is no CPU fallback:
+if the library is missing, importing this module raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD_DIR = os.path.join(os.path.dirname(_HERE), "_build")
+LIB_PATH = os.path.join(BUILD_DIR, "libnuc34.so")
+
+OK, E_OOM, E_OVERFLOW, E_HIP, E_INVALID, E_IO, E_STATE = range(7)
+DIRECT, ONDEMAND, AUTO = 0, 1, 2
+VARIANTS = {"direct": DIRECT, "ondemand": ONDEMAND, "auto": AUTO}
+
+
+class Nuc34Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint64), ("m", C.c_uint64), ("triangles", C.c_uint64), ("four_cliques", C.c_uint64),
+        ("incidence_entries", C.c_uint64), ("rounds", C.c_uint64), ("unpeeled", C.c_uint64),
+        ("peak_bytes", C.c_uint64), ("k_max", C.c_int32), ("variant", C.c_int32),
+        ("ms_orient", C.c_double), ("ms_triangles", C.c_double), ("ms_incidence", C.c_double),
+        ("ms_support", C.c_double), ("ms_k4_list", C.c_double), ("ms_peel", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libnuc34.so not built at {LIB_PATH}; run __graft_entry__.build() or make -C gpu-nucleus34_amd")
+    lib = C.CDLL(LIB_PATH)
+    u32p, i32p, u64p = C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(C.c_uint64)
+    vp = C.c_void_p
+    sig = {
+        "nuc34_graph_load": (C.c_int, [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t]),
+        "nuc34_graph_from_pairs": (C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(vp)]),
+        "nuc34_graph_info": (C.c_int, [vp, u32p, u64p, C.POINTER(C.c_int)]),
+        "nuc34_graph_copy": (C.c_int, [vp, u32p, u32p]),
+        "nuc34_graph_orient_host": (C.c_int, [vp]),
+        "nuc34_graph_free": (None, [vp]),
+        "nuc34_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "nuc34_ctx_destroy": (None, [vp]),
+        "nuc34_last_error": (C.c_char_p, [vp]),
+        "nuc34_load_csr": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_uint64, C.c_int]),
+        "nuc34_graph_dims": (C.c_int, [vp, u32p, u64p]),
+        "nuc34_copy_oriented": (C.c_int, [vp, u32p, u32p]),
+        "nuc34_triangles": (C.c_int, [vp, C.c_uint32, u64p]),
+        "nuc34_copy_triangles": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p]),
+        "nuc34_support": (C.c_int, [vp, u64p]),
+        "nuc34_copy_support": (C.c_int, [vp, i32p]),
+        "nuc34_decompose": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(Stats)]),
+        "nuc34_copy_k": (C.c_int, [vp, i32p]),
+        "nuc34_copy_fourcliques": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p]),
+        "nuc34_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+        "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+
+def lib() -> C.CDLL:
+    return _lib
+
+
+def _u32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+@dataclass
+class HostGraph:
+    """A dataset as the reference loads it: undirected CSR (oriented=False) or a
+    pre-oriented .grh CSR (oriented=True)."""
+    O: np.ndarray
+    E: np.ndarray
+    oriented: bool
+
+    @property
+    def n(self) -> int:
+        return len(self.O) - 1
+
+    @property
+    def m(self) -> int:
+        return len(self.E)
+
+
+def _take_graph(h) -> HostGraph:
+    n, m, o = C.c_uint32(), C.c_uint64(), C.c_int()
+    _lib.nuc34_graph_info(h, C.byref(n), C.byref(m), C.byref(o))
+    O = np.empty(n.value + 1, dtype=np.uint32)
+    E = np.empty(max(m.value, 1), dtype=np.uint32)
+    _lib.nuc34_graph_copy(h, _u32(O), _u32(E))
+    _lib.nuc34_graph_free(h)
+    return HostGraph(O, E[: m.value].copy(), bool(o.value))
+
+
+def load_dataset(path: str) -> HostGraph:
+    """graph_t::from_edges / deserialize by extension (.mtx, .edges, .txt, .grh)."""
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = _lib.nuc34_graph_load(os.fsencode(path), C.byref(h), err, 512)
+    if rc != OK:
+        raise Nuc34Error(rc, err.value.decode(errors="replace"))
+    return _take_graph(h)
+
+
+def graph_from_pairs(src, dst) -> HostGraph:
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    h = C.c_void_p()
+    rc = _lib.nuc34_graph_from_pairs(_u32(src), _u32(dst), len(src), C.byref(h))
+    if rc != OK:
+        raise Nuc34Error(rc, "graph_from_pairs failed")
+    return _take_graph(h)
+
+
+def device_info(device: int = 0) -> dict:
+    buf = C.create_string_buffer(4096)
+    rc = _lib.nuc34_device_info(device, buf, 4096)
+    if rc != OK:
+        raise Nuc34Error(rc, "no HIP device")
+    return dict(line.split("=", 1) for line in buf.value.decode().splitlines() if "=" in line)
+
+
+class Context:
+    """A device context: owns every device buffer of one decomposition."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._h = C.c_void_p()
+        rc = _lib.nuc34_ctx_create(device, hbm_budget, C.byref(self._h))
+        if rc != OK:
+            raise Nuc34Error(rc, "nuc34_ctx_create failed")
+
+    def close(self):
+        if self._h:
+            _lib.nuc34_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != OK:
+            raise Nuc34Error(rc, _lib.nuc34_last_error(self._h).decode(errors="replace"))
+
+    def load_csr(self, O, E, oriented: bool):
+        O = np.ascontiguousarray(O, dtype=np.uint32)
+        E = np.ascontiguousarray(E, dtype=np.uint32)
+        Ep = E if len(E) else np.zeros(1, dtype=np.uint32)
+        self._check(_lib.nuc34_load_csr(self._h, _u32(O), _u32(Ep), len(O) - 1, len(E), int(bool(oriented))))
+
+    def load_graph(self, g: HostGraph):
+        self.load_csr(g.O, g.E, g.oriented)
+
+    def dims(self):
+        n, m = C.c_uint32(), C.c_uint64()
+        self._check(_lib.nuc34_graph_dims(self._h, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
+    def oriented_csr(self):
+        n, m = self.dims()
+        O = np.empty(n + 1, dtype=np.uint32)
+        E = np.empty(max(m, 1), dtype=np.uint32)
+        self._check(_lib.nuc34_copy_oriented(self._h, _u32(O), _u32(E)))
+        return O, E[:m]
+
+    def triangles(self, parts: int = 1) -> int:
+        T = C.c_uint64()
+        self._check(_lib.nuc34_triangles(self._h, parts, C.byref(T)))
+        return T.value
+
+    def copy_triangles(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        T = self.triangles()
+        if count is None:
+            count = T - first
+        out = np.empty((max(count, 1), 3), dtype=np.uint32)
+        self._check(_lib.nuc34_copy_triangles(self._h, first, count, _u32(out)))
+        return out[:count]
+
+    def support(self) -> int:
+        Q = C.c_uint64()
+        self._check(_lib.nuc34_support(self._h, C.byref(Q)))
+        return Q.value
+
+    def copy_support(self) -> np.ndarray:
+        T = self.triangles()
+        out = np.empty(max(T, 1), dtype=np.int32)
+        self._check(_lib.nuc34_copy_support(self._h, _i32(out)))
+        return out[:T]
+
+    def decompose(self, variant="auto", parts: int = 1) -> dict:
+        v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        st = Stats()
+        self._check(_lib.nuc34_decompose(self._h, v, parts, C.byref(st)))
+        return st.as_dict()
+
+    def copy_k(self) -> np.ndarray:
+        T = self.triangles()
+        out = np.empty(max(T, 1), dtype=np.int32)
+        self._check(_lib.nuc34_copy_k(self._h, _i32(out)))
+        return out[:T]
+
+    def copy_fourcliques(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.stats()["four_cliques"] - first
+        out = np.empty((max(count, 1), 4), dtype=np.uint32)
+        self._check(_lib.nuc34_copy_fourcliques(self._h, first, count, _u32(out)))
+        return out[:count]
+
+    def stats(self) -> dict:
+        st = Stats()
+        self._check(_lib.nuc34_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+
+def decompose_graph(g: HostGraph, variant="auto", parts: int = 1, device: int = 0, hbm_budget: int = 0):
+    """One-shot: returns (triangles (T,3), k (T,), stats)."""
+    with Context(device, hbm_budget) as ctx:
+        ctx.load_graph(g)
+        st = ctx.decompose(variant, parts)
+        return ctx.copy_triangles(), ctx.copy_k(), st

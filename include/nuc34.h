@@ -1,0 +1,127 @@
+/*
+ * nuc34 — C ABI of the MI355X (gfx950) (3,4)-nucleus decomposition.
+ *
+ * Plain pointers and sizes only; device memory never crosses the boundary.
+ * Every entry point returns a status code (NUC34_OK on success); the message
+ * of the last failure on a context is available from nuc34_last_error().
+ *
+ * Each entry point replaces a piece of the reference's operator surface
+ * (paths relative to the reference tree):
+ *   nuc34_graph_load        graph_t::from_edges / deserialize + extension
+ *                           dispatch   (nucleus/graph/graph.hpp:286-411, 491-536;
+ *                           nucleus/cuNucleus34.cu:235-256)
+ *   nuc34_load_csr          host make_oriented + H2D of O, D, E
+ *                           (graph.hpp:176-218; cuNucleus34.cu:290-293)
+ *   nuc34_triangles         cu::compute_triangles / _segmented / _parted +
+ *                           thrust::sort(tri_t)  (nucleus/utils.cuh:1282, 1411, 1880,
+ *                           1897-1912; cuNucleus34_ondemand.cu:497-500)
+ *   nuc34_support           edge->triangle incidence + count_support
+ *                           (cuNucleus34_ondemand.cu:502-584)
+ *   nuc34_decompose         direct: cu::compute_fourcliques + incidence CSR +
+ *                           peel_fourcliques loop (utils.cuh:2173; cuNucleus34.cu:511-714);
+ *                           on-demand: mark_dying/debit_partners loop
+ *                           (cuNucleus34_ondemand.cu:599-690)
+ *   nuc34_copy_*            the D2H copies of d_tris, d_kv, d_quds and the
+ *                           cudaMemcpyFromSymbol counters (cuNucleus34.cu:391, 556, 716-721)
+ */
+#ifndef NUC34_H_
+#define NUC34_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NUC34_OK = 0,
+  NUC34_E_OOM = 1,      /* device memory or HBM budget exhausted */
+  NUC34_E_OVERFLOW = 2, /* a count exceeds the 32-bit id range it indexes */
+  NUC34_E_HIP = 3,      /* HIP runtime error */
+  NUC34_E_INVALID = 4,  /* bad argument or malformed input */
+  NUC34_E_IO = 5,       /* dataset unreadable or unknown format */
+  NUC34_E_STATE = 6     /* call out of order (e.g. no graph loaded) */
+};
+
+enum { NUC34_DIRECT = 0, NUC34_ONDEMAND = 1, NUC34_AUTO = 2 };
+
+typedef struct nuc34_ctx nuc34_ctx;
+typedef struct nuc34_graph nuc34_graph;
+
+typedef struct nuc34_stats {
+  uint64_t n;                 /* vertices */
+  uint64_t m;                 /* oriented edges */
+  uint64_t triangles;         /* T */
+  uint64_t four_cliques;      /* Q */
+  uint64_t incidence_entries; /* 3T edge->triangle entries */
+  uint64_t rounds;            /* peel rounds */
+  uint64_t unpeeled;          /* triangles left without k (0 on success) */
+  uint64_t peak_bytes;        /* peak device bytes held by the context */
+  int32_t k_max;
+  int32_t variant;            /* NUC34_DIRECT or NUC34_ONDEMAND: what ran */
+  double ms_orient;           /* device times from HIP events, milliseconds */
+  double ms_triangles;
+  double ms_incidence;
+  double ms_support;
+  double ms_k4_list;          /* direct: K4 list + triangle->K4 incidence */
+  double ms_peel;
+  double ms_total;            /* triangles .. peel (orientation excluded) */
+} nuc34_stats;
+
+/* ---- host-side datasets (reference loaders, exact semantics) ------------ */
+
+/* Loads .mtx / .edges / .txt (undirected result of from_edges) or .grh (stored,
+ * already oriented) by extension. */
+int nuc34_graph_load(const char* path, nuc34_graph** out, char* err, size_t errlen);
+/* Undirected graph from raw (src[i], dst[i]) pairs, as from_edges builds it
+ * after its header line. */
+int nuc34_graph_from_pairs(const uint32_t* src, const uint32_t* dst, uint64_t count, nuc34_graph** out);
+int nuc34_graph_info(const nuc34_graph* g, uint32_t* n, uint64_t* m, int* oriented);
+/* O: n+1 offsets, E: m vertex ids. */
+int nuc34_graph_copy(const nuc34_graph* g, uint32_t* O, uint32_t* E);
+/* Host orientation (make_oriented) of an undirected dataset, in place. */
+int nuc34_graph_orient_host(nuc34_graph* g);
+void nuc34_graph_free(nuc34_graph* g);
+
+/* ---- device context ------------------------------------------------------ */
+
+/* hbm_budget: bytes the context may hold on the device (0 = no limit). */
+int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out);
+void nuc34_ctx_destroy(nuc34_ctx* ctx);
+const char* nuc34_last_error(const nuc34_ctx* ctx);
+
+/* Uploads a CSR. oriented == 0: undirected (both directions, lists sorted by
+ * id); the device orients it with the degree order. oriented != 0: used as
+ * given (lists sorted, acyclic). Resets every later stage. */
+int nuc34_load_csr(nuc34_ctx* ctx, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int oriented);
+int nuc34_graph_dims(const nuc34_ctx* ctx, uint32_t* n, uint64_t* m_oriented);
+int nuc34_copy_oriented(nuc34_ctx* ctx, uint32_t* O, uint32_t* E);
+
+/* Triangles in the reference's canonical (a,b) order; `parts` >= 1 splits the
+ * enumeration into that many vertex ranges (same result for every value). */
+int nuc34_triangles(nuc34_ctx* ctx, uint32_t parts, uint64_t* T);
+/* abc: 3 * count entries (a, b, c) for triangles [first, first + count). */
+int nuc34_copy_triangles(nuc34_ctx* ctx, uint64_t first, uint64_t count, uint32_t* abc);
+
+/* Edge->triangle incidence and per-triangle K4 support; Q = sum / 4. */
+int nuc34_support(nuc34_ctx* ctx, uint64_t* Q);
+int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
+
+/* Runs the missing stages and the peel. variant: NUC34_DIRECT, NUC34_ONDEMAND or
+ * NUC34_AUTO (direct when its K4 list fits the budget). */
+int nuc34_decompose(nuc34_ctx* ctx, int variant, uint32_t parts, nuc34_stats* stats);
+/* k for every triangle in (a,b) order (0 for triangles in no K4). */
+int nuc34_copy_k(nuc34_ctx* ctx, int32_t* k);
+/* Direct variant only: K4s [first, first+count) as 4 triangle ids each. */
+int nuc34_copy_fourcliques(nuc34_ctx* ctx, uint64_t first, uint64_t count, uint32_t* quds);
+int nuc34_get_stats(const nuc34_ctx* ctx, nuc34_stats* stats);
+
+/* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
+int nuc34_device_info(int device, char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUC34_H_ */
