@@ -1,0 +1,171 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the golden
+vectors. Integer results must be bit-exact: triangle identities and order,
+support, T, Q, K_max and every k."""
+import hashlib
+import itertools
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import nuc34
+import oracle34
+from conftest import BUILD, GOLDEN, gen
+
+pytestmark = pytest.mark.gpu
+EXPECTED = json.load(open(os.path.join(GOLDEN, "expected.json")))
+HASHES = json.load(open(os.path.join(GOLDEN, "hashes.json")))
+
+
+def kfile_sha(tris, k):
+    h = hashlib.sha256()
+    chunk = 1 << 20
+    for i in range(0, len(k), chunk):
+        t = tris[i:i + chunk].tolist()
+        kk = k[i:i + chunk].tolist()
+        h.update("".join(f"({a},{b},{c})\t{x}\n" for (a, b, c), x in zip(t, kk)).encode())
+    return h.hexdigest()
+
+
+def gpu_run(g, variant, parts=1, budget=0):
+    with nuc34.Context(0, budget) as ctx:
+        ctx.load_graph(g)
+        st = ctx.decompose(variant, parts)
+        return ctx.copy_triangles(), ctx.copy_support(), ctx.copy_k(), st
+
+
+@pytest.mark.parametrize("name", sorted(n for n in EXPECTED if "rows" in EXPECTED[n]))
+@pytest.mark.parametrize("variant,parts", [("direct", 1), ("ondemand", 1), ("direct", 3), ("auto", 2)])
+def test_golden(name, variant, parts):
+    print(f"[golden] {name} {variant} parts={parts}", flush=True)
+    exp = EXPECTED[name]
+    g = nuc34.load_dataset(os.path.join(GOLDEN, name))
+    tris, sup, k, st = gpu_run(g, variant, parts)
+    rows = [[int(a), int(b), int(c), int(x)] for (a, b, c), x in zip(tris.tolist(), k.tolist())]
+    assert st["triangles"] == exp["T"] and st["four_cliques"] == exp["Q"]
+    assert rows == exp["rows"]
+    if exp["Q"]:
+        assert st["k_max"] == exp["K_max"]
+    assert st["unpeeled"] == 0
+
+
+CASES = [
+    ("er-p", 300, 0.2, 1), ("er-deg", 3000, 40, 2), ("ba", 4000, 12, 3), ("ba", 2000, 3, 4),
+    ("rmat", 11, 16, 0.57, 0.19, 0.19, 5), ("rmat", 12, 16, 0.57, 0.19, 0.19, 6),
+    ("rmat", 12, 32, 0.45, 0.15, 0.15, 7), ("rmat", 10, 16, 0.57, 0.19, 0.19, 8, 6, 25),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(map(str, c)))
+def test_generated_vs_oracle(tmp_path, case):
+    print(f"[generated] {case}", flush=True)
+    p = gen(*case[:7], out=tmp_path / "g.edges") if case[0] != "rmat" or len(case) == 7 else None
+    if p is None:
+        subprocess.run([os.path.join(BUILD, "nuc34_gen"), *map(str, case[:7]), str(tmp_path / "g.edges"),
+                        *map(str, case[7:])], check=True, stdout=subprocess.DEVNULL)
+        p = str(tmp_path / "g.edges")
+    ref = oracle34.OracleGraph.load(p, orient=True).run("ondemand", 8)
+    g = nuc34.load_dataset(p)
+    for variant in ("direct", "ondemand"):
+        tris, sup, k, st = gpu_run(g, variant, parts=1 if variant == "direct" else 4)
+        assert st["triangles"] == ref["T"] and st["four_cliques"] == ref["Q"], variant
+        assert np.array_equal(tris, ref["tris"]), variant
+        assert np.array_equal(sup, ref["support"]), variant
+        assert np.array_equal(k, ref["k"]), variant
+        assert st["k_max"] == ref["K_max"] and st["rounds"] == ref["rounds"], variant
+
+
+def test_device_orientation_matches_reference_order(tmp_path):
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 9, out=tmp_path / "g.edges")
+    g = nuc34.load_dataset(p)
+    with nuc34.Context() as ctx:
+        ctx.load_graph(g)
+        O, E = ctx.oriented_csr()
+    Oo, Eo = oracle34.OracleGraph.load(p, orient=True).csr()
+    assert np.array_equal(O, Oo) and np.array_equal(E, Eo)
+
+
+@pytest.mark.parametrize("n", [4, 6, 11, 40])
+def test_clique_closed_form(n):
+    pairs = np.array(list(itertools.combinations(range(n), 2)), dtype=np.uint32)
+    g = nuc34.graph_from_pairs(pairs[:, 0], pairs[:, 1])
+    for variant in ("direct", "ondemand"):
+        tris, sup, k, st = gpu_run(g, variant)
+        assert st["triangles"] == n * (n - 1) * (n - 2) // 6
+        assert st["four_cliques"] == n * (n - 1) * (n - 2) * (n - 3) // 24
+        assert (k == n - 3).all() and st["k_max"] == n - 3
+
+
+def test_no_triangles_and_empty():
+    for pairs in ([(0, i) for i in range(1, 50)], [(0, 1)]):
+        a = np.array(pairs, dtype=np.uint32)
+        g = nuc34.graph_from_pairs(a[:, 0], a[:, 1])
+        with nuc34.Context() as ctx:
+            ctx.load_graph(g)
+            assert ctx.triangles() == 0
+            st = ctx.decompose("auto")
+            assert st["triangles"] == 0 and st["four_cliques"] == 0 and len(ctx.copy_k()) == 0
+
+
+def test_budget_selects_ondemand_and_reports_oom(tmp_path):
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 10, out=tmp_path / "g.edges")
+    g = nuc34.load_dataset(p)
+    with nuc34.Context(0, 64 << 20) as ctx:  # room for O(T) state, not the K4 list
+        ctx.load_graph(g)
+        st = ctx.decompose("auto")
+        assert st["variant"] == nuc34.ONDEMAND
+        k_small = ctx.copy_k()
+    _, _, k_big, st2 = gpu_run(g, "direct")
+    assert st2["variant"] == nuc34.DIRECT and np.array_equal(k_small, k_big)
+    with nuc34.Context(0, 1 << 16) as ctx:
+        with pytest.raises(nuc34.Nuc34Error) as ei:
+            ctx.load_graph(g)
+            ctx.decompose("direct")
+        assert ei.value.code == nuc34.E_OOM and "budget" in str(ei.value)
+
+
+@pytest.mark.parametrize("preset", ["er1000", "rmat16"])
+def test_full_size_hash(tmp_path, preset):
+    """Full configurations: the k file must hash to the value the reference's
+    own host path produced; supports must sum to 4Q; k <= support."""
+    p = gen("preset", preset, out=tmp_path / f"{preset}.edges")
+    exp = HASHES[preset]
+    g = nuc34.load_dataset(p)
+    ks = {}
+    for variant, parts in (("direct", 1), ("ondemand", 1), ("direct", 4)):
+        print(f"[full] {preset} {variant} parts={parts}", flush=True)
+        tris, sup, k, st = gpu_run(g, variant, parts)
+        assert st["triangles"] == exp["T"] and st["four_cliques"] == exp["Q"] and st["k_max"] == exp["K_max"]
+        assert int(sup.astype(np.int64).sum()) == 4 * exp["Q"]
+        assert (k <= sup).all() and (k >= 0).all()
+        assert kfile_sha(tris, k) == exp["kfile_sha256"], (variant, parts)
+        ks[(variant, parts)] = k
+    first = next(iter(ks.values()))
+    assert all(np.array_equal(first, v) for v in ks.values())
+
+
+def test_cli_matches_golden(tmp_path):
+    for exe, name in (("nucleus34", "rmat8.edges"), ("nucleus34_ondemand", "rmat7k8.edges"),
+                      ("nucleus34", "er40.grh"), ("nucleus34", "star.edges"), ("nucleus34_ondemand", "wheel.edges")):
+        exp = EXPECTED[name]
+        out = tmp_path / f"{name}.k"
+        args = [os.path.join(BUILD, exe), os.path.join(GOLDEN, name)]
+        args += (["2", str(out)] if exe == "nucleus34" else [str(out)])
+        r = subprocess.run(args, capture_output=True, text=True, check=True)
+        rec = json.loads(r.stdout)["dataset"]
+        assert "error" not in rec, rec
+        assert rec["triangle_count"] == exp["T"]
+        if exp["T"] == 0:
+            assert "four_clique_count" not in rec
+            continue
+        assert rec["four_clique_count"] == exp["Q"]
+        if exp["Q"] == 0:
+            assert "K_max" not in rec
+            continue
+        assert rec["K_max"] == exp["K_max"] and rec["K_count"] == exp["T"]
+        assert rec["K"] == [r_[3] for r_ in exp["rows"][:5]]
+        rows = [[*map(int, line.split("\t")[0].strip("()").split(",")), int(line.split("\t")[1])]
+                for line in out.read_text().splitlines()]
+        assert rows == exp["rows"]
