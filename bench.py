@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Headline benchmark: (3,4)-nucleus decomposition on one MI355X.
+
+Workload (default): R-MAT scale 16, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05),
+seed 1, symmetrised and deduplicated (15.5 M triangles, 287 M four-cliques),
+direct variant with the per-triangle K4 lists materialised in HBM.
+
+A step = one decomposition from the oriented CSR resident in HBM to k for every
+triangle resident in HBM: triangle enumeration + bit-packed incidence, K4
+support, K4 lists, peel. Between steps the context drops the derived stages
+(buffers return to its reuse cache, so the timed steps allocate nothing).
+value = four-cliques / second = Q / (mean step time).
+
+Prints one JSON line with the driver's contract, plus:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average launch time (HIP events on the context stream);
+  cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
+                the reference headers) timed on this host's cores on a bounded
+                sample (R-MAT scale 14, same generator parameters).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gpu-nucleus34_amd")
+GEN = os.path.join(PKG, "_build", "nuc34_gen")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+WORKLOADS = {
+    "rmat16": {"gen": ["preset", "rmat16"], "variant": "direct", "parts": 1,
+               "desc": "R-MAT scale 16, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 1, direct variant"},
+    "rmat16-parts4": {"gen": ["preset", "rmat16"], "variant": "direct", "parts": 4,
+                      "desc": "R-MAT scale 16, edge factor 16, seed 1, direct variant, parts=4"},
+    "rmat16-ondemand": {"gen": ["preset", "rmat16"], "variant": "ondemand", "parts": 1,
+                        "desc": "R-MAT scale 16, edge factor 16, seed 1, on-demand variant"},
+    "er1000": {"gen": ["preset", "er1000"], "variant": "direct", "parts": 1,
+               "desc": "Erdos-Renyi G(1000, 0.1), seed 0, direct variant"},
+    "rmat20k30": {"gen": ["preset", "rmat20k30"], "variant": "ondemand", "parts": 1,
+                  "desc": "R-MAT scale 20, edge factor 16, seed 2, + 64 planted K30, on-demand variant"},
+}
+
+
+def graph_stats(O: np.ndarray, E: np.ndarray) -> dict:
+    d = np.diff(O.astype(np.int64))
+    P = int(d[E].sum()) if len(E) else 0  # length-2 paths: sum over edges of d+(head)
+    return {"n": len(d), "m": len(E), "P": P, "sum_dplus2": int((d * d).sum())}
+
+
+def algorithmic_bytes(kernel: str, g: dict, st: dict) -> float:
+    """Bytes an ideal implementation of `kernel` must move per launch."""
+    T, Q, m, n = st["triangles"], st["four_cliques"], g["m"], g["n"]
+    if kernel == "tri_enum_bits":
+        # per edge: src, head, O[u], O[u+1], O[v], O[v+1]; each candidate w of
+        # N+(v); the owner lists; one 4-byte read-modify-write per role bit
+        return 24 * m + 4 * g["P"] + 4 * m + 3 * 8 * T
+    if kernel == "tri_enum_emit":
+        return 24 * m + 4 * g["P"] + 4 * m + T * (12 + 8)
+    if kernel == "support_bits":
+        return T * (12 + 12) + 2 * 4 * m  # triangle record in, three counts out, rows read once
+    if kernel == "k4_owned":
+        return T * 12 + Q * 8
+    if kernel == "k4_place":
+        return Q * (8 + 4 * 12)  # owned-K4 record in, four 12-byte list entries out
+    if kernel == "peel_rounds":
+        return 4 * Q * 12 + 3 * Q * 8 + T * 20  # every list once, debits, frontier + offsets
+    return 0.0
+
+
+def run_cpu_baseline(tmp: str, threads: int) -> dict:
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref34")
+    kind = "reference"
+    if not os.path.exists(ref):
+        ref = os.path.join(ROOT, "oracle", "_build", "oracle34")
+        kind = "port"
+        if not os.path.exists(ref):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    path = os.path.join(tmp, "sample.edges")
+    subprocess.run([GEN, "rmat", "14", "16", "0.57", "0.19", "0.19", "1", path], check=True,
+                   stdout=subprocess.DEVNULL)
+    args = [ref, path, "", str(threads)] if kind == "reference" else [ref, path, "--parts", str(threads)]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run(args, check=True, capture_output=True, text=True, env=env).stdout
+    rec = json.loads(out.strip().splitlines()[-1])
+    secs = sum(rec.get(k, 0.0) for k in ("tri_s", "tri_graph_s", "qud_s", "qud_graph_s", "peel_s"))
+    return {"value": rec["Q"] / secs, "unit": "4-cliques/s", "cores": threads, "kind": kind,
+            "sample": f"R-MAT scale 14, edge factor 16, same (a,b,c,d), seed 1: T={rec['T']}, Q={rec['Q']}, "
+                      f"K_max={rec['K_max']}; decomposition {secs:.2f} s (load excluded); "
+                      f"{'reference host path compiled from cliques.hpp+bucket_ex.hpp (serial pstl: no TBB)' if kind == 'reference' else 'oracle restatement'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat16", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (graphs shard by graph or by "
+              "vertex range as DESIGN.md describes, but this build runs on one device)", file=sys.stderr)
+        sys.exit(2)
+    import nuc34
+
+    wl = WORKLOADS[args.workload]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "graph.edges")
+        subprocess.run([GEN, *wl["gen"], path], check=True, stdout=subprocess.DEVNULL)
+        g = nuc34.load_dataset(path)
+        ctx = nuc34.Context(0)
+        ctx.load_graph(g)
+        O, E = ctx.oriented_csr()
+        gs = graph_stats(O, E)
+        for _ in range(args.warmup):
+            ctx.reset()
+            ctx.decompose(wl["variant"], wl["parts"])
+        steps = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.reset()
+            steps.append(ctx.decompose(wl["variant"], wl["parts"]))
+        t1 = time.perf_counter()
+        ctx.close()
+        cpu = None if args.no_cpu_baseline else run_cpu_baseline(tmp, args.cpu_threads)
+
+    st = steps[-1]
+    ms = (t1 - t0) / args.steps * 1e3
+    value = st["four_cliques"] / (ms / 1e3)
+    # dominant kernel over the timed steps
+    kern = {}
+    for s_ in steps:
+        for name, k in s_["kernels"].items():
+            a = kern.setdefault(name, {"ms": 0.0, "launches": 0})
+            a["ms"] += k["ms"]
+            a["launches"] += k["launches"]
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    per_launch_ms = kern[dom]["ms"] / kern[dom]["launches"]
+    launches_per_step = kern[dom]["launches"] / args.steps
+    bytes_per_launch = algorithmic_bytes(dom, gs, st) / launches_per_step
+    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "hashes.json"))).get(wl["gen"][-1], {})
+    parity = all(st[k] == golden[g_] for k, g_ in (("triangles", "T"), ("four_cliques", "Q"), ("k_max", "K_max"))
+                 if g_ in golden) if golden else None
+    rec = {
+        "metric": "4-cliques/s (decomposition wall time, bit-exact k)",
+        "value": value,
+        "unit": "4-cliques/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded generator, gpu-nucleus34_amd/_build/nuc34_gen)",
+        "config": {"workload": wl["desc"], "variant": wl["variant"], "parts": wl["parts"],
+                   "n": gs["n"], "m_oriented": gs["m"], "triangles": st["triangles"],
+                   "four_cliques": st["four_cliques"], "K_max": st["k_max"], "rounds": st["rounds"],
+                   "counts_match_reference": parity,
+                   "phase_ms": {k: round(st[k], 3) for k in ("ms_triangles", "ms_support", "ms_k4_list",
+                                                           "ms_incidence", "ms_peel", "ms_total")},
+                   "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,21 +34,35 @@ struct nuc34_ctx {
   uint32_t n = 0;
   uint64_t m = 0;
   DevBuf O, E, src;
-  // triangles
+  // triangles and the bit-packed incidence
   bool has_tris = false;
   uint64_t T = 0;
-  DevBuf cntA, cntB, cntC, offA, triA, triBC;
-  // incidence + support
+  DevBuf IO, IE, IEe, inpos;        // in-CSR of the oriented graph
+  DevBuf rowA, rowC, Aout, Ain, Cm;  // bit rows
+  DevBuf offA, triA, triBC;
+  // support
   bool has_sup = false;
   uint64_t Q = 0;
-  DevBuf incO, third, tri, sup, own, bigq, small;  // small: bigcount(u32) + total(u64)
-  // direct K4 list
+  DevBuf sup, supUp, own, small;  // small: counters
+  // third-vertex-sorted incidence (on-demand re-enumeration)
+  bool has_inc = false;
+  DevBuf incO, third, tri, bigq;
+  // direct: per-triangle K4 partner lists
   bool has_k4 = false;
-  DevBuf k4off, qud, koff, incK;
+  DevBuf koff, lists;
   // peel result
   bool has_k = false;
   DevBuf kv;
   nuc34_stats st{};
+  // per-kernel event timers
+  struct KTimer {
+    const char* name = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    uint64_t launches = 0;
+    double ms = 0;
+    bool pending = false;
+  };
+  KTimer kt[NUC34_MAX_KERNEL_STATS];
 };
 
 namespace {
@@ -87,6 +102,61 @@ double timed(nuc34_ctx* c, F&& f) {
   return ms;
 }
 
+enum KernelId { K_TRI_BITS, K_TRI_EMIT, K_SUPPORT, K_K4_OWNED, K_K4_PLACE, K_PEEL, K_COUNT };
+const char* const kKernelNames[K_COUNT] = {"tri_enum_bits", "tri_enum_emit", "support_bits",
+                                          "k4_owned",      "k4_place",      "peel_rounds"};
+
+// Records events around `f` on the context stream; read back by ktime_collect.
+template <typename F>
+void ktime(nuc34_ctx* c, int id, uint64_t launches, F&& f) {
+  auto& k = c->kt[id];
+  if (k.pending) {
+    NUC34_HIP(hipEventSynchronize(k.b));
+    float ms = 0;
+    NUC34_HIP(hipEventElapsedTime(&ms, k.a, k.b));
+    k.ms += ms;
+    k.pending = false;
+  }
+  if (!k.a) {
+    NUC34_HIP(hipEventCreate(&k.a));
+    NUC34_HIP(hipEventCreate(&k.b));
+  }
+  k.name = kKernelNames[id];
+  NUC34_HIP(hipEventRecord(k.a, c->s));
+  f();
+  NUC34_HIP(hipEventRecord(k.b, c->s));
+  k.launches += launches;
+  k.pending = true;
+}
+
+void ktime_reset(nuc34_ctx* c) {
+  for (auto& k : c->kt) {
+    k.launches = 0;
+    k.ms = 0;
+    k.pending = false;
+  }
+}
+
+void ktime_collect(nuc34_ctx* c) {
+  c->st.n_kernels = 0;
+  for (int id = 0; id < K_COUNT; ++id) {
+    auto& k = c->kt[id];
+    if (k.pending) {
+      NUC34_HIP(hipEventSynchronize(k.b));
+      float ms = 0;
+      NUC34_HIP(hipEventElapsedTime(&ms, k.a, k.b));
+      k.ms += ms;
+      k.pending = false;
+    }
+    if (!k.launches) continue;
+    nuc34_kernel_stat& o = c->st.kernels[c->st.n_kernels++];
+    std::memset(o.name, 0, sizeof(o.name));
+    std::strncpy(o.name, kKernelNames[id], sizeof(o.name) - 1);
+    o.ms = k.ms;
+    o.launches = k.launches;
+  }
+}
+
 template <typename T>
 T read_scalar(nuc34_ctx* c, const void* dev) {
   T v{};
@@ -100,25 +170,25 @@ void zero(nuc34_ctx* c, DevBuf& b) {
 }
 
 void reset_stages(nuc34_ctx* c, int from) {
-  // from: 0 graph, 1 triangles, 2 support, 3 k4, 4 k
+  // from: 0 graph, 1 triangles, 2 support, 3 k4 lists / incidence, 4 k
   if (from <= 4) {
     c->kv.release();
     c->has_k = false;
   }
   if (from <= 3) {
-    c->k4off.release();
-    c->qud.release();
-    c->koff.release();
-    c->incK.release();
+    for (DevBuf* b : {&c->koff, &c->lists, &c->incO, &c->third, &c->tri, &c->bigq}) b->release();
     c->has_k4 = false;
+    c->has_inc = false;
   }
   if (from <= 2) {
-    for (DevBuf* b : {&c->incO, &c->third, &c->tri, &c->sup, &c->own, &c->bigq, &c->small}) b->release();
+    for (DevBuf* b : {&c->sup, &c->supUp, &c->own, &c->small}) b->release();
     c->has_sup = false;
     c->Q = 0;
   }
   if (from <= 1) {
-    for (DevBuf* b : {&c->cntA, &c->cntB, &c->cntC, &c->offA, &c->triA, &c->triBC}) b->release();
+    for (DevBuf* b : {&c->IO, &c->IE, &c->IEe, &c->inpos, &c->rowA, &c->rowC, &c->Aout, &c->Ain, &c->Cm, &c->offA,
+                      &c->triA, &c->triBC})
+      b->release();
     c->has_tris = false;
     c->T = 0;
   }
@@ -152,48 +222,112 @@ std::vector<std::pair<uint64_t, uint64_t>> edge_parts(nuc34_ctx* c, uint32_t par
   return r;
 }
 
+BitRows bitrows(nuc34_ctx* c) {
+  return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<uint32_t>(),
+                 c->Cm.as<uint32_t>()};
+}
+
+TriView triview(nuc34_ctx* c) {
+  return TriView{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->T, c->incO.as<uint64_t>(),
+                 c->third.as<uint32_t>(), c->tri.as<uint32_t>()};
+}
+
 void run_triangles(nuc34_ctx* c, uint32_t parts) {
   const uint64_t m = c->m;
+  const uint32_t n = c->n;
   const auto ranges = edge_parts(c, parts);
   c->st.ms_triangles = timed(c, [&] {
-    c->cntA = c->alloc.get(m * 4 + 4, "triangle a-role counts");
-    c->cntB = c->alloc.get(m * 4 + 4, "triangle b-role counts");
-    c->cntC = c->alloc.get(m * 4 + 4, "triangle c-role counts");
-    zero(c, c->cntA);
-    zero(c, c->cntB);
-    zero(c, c->cntC);
-    for (const auto& r : ranges)
-      launch_tri_count(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second,
-                       c->cntA.as<uint32_t>(), c->cntB.as<uint32_t>(), c->cntC.as<uint32_t>(), c->s);
+    // in-CSR: the c-role rows are indexed by position in the source's in-list
+    c->IO = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "in-offsets");
+    c->IE = c->alloc.get(m * 4 + 4, "in-lists");
+    c->IEe = c->alloc.get(m * 4 + 4, "in-list edges");
+    c->inpos = c->alloc.get(m * 4 + 4, "in-list positions");
+    build_in_csr(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), n, m, c->IO.as<uint32_t>(),
+                 c->IE.as<uint32_t>(), c->IEe.as<uint32_t>(), c->inpos.as<uint32_t>(), c->alloc, c->s);
+    // row layout
+    c->rowA = c->alloc.get((m + 1) * 8, "row offsets");
+    c->rowC = c->alloc.get((m + 1) * 8, "row offsets");
+    {
+      DevBuf wa = c->alloc.get(m * 4 + 4, "row words");
+      DevBuf wc = c->alloc.get(m * 4 + 4, "row words");
+      launch_row_words(c->O.as<uint32_t>(), c->IO.as<uint32_t>(), c->src.as<uint32_t>(), m, wa.as<uint32_t>(),
+                       wc.as<uint32_t>(), c->s);
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(wa.as<uint32_t>(), c->rowA.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
+      exclusive_scan<uint32_t, uint64_t>(wc.as<uint32_t>(), c->rowC.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
+    }
+    const uint64_t WA = read_scalar<uint64_t>(c, c->rowA.as<uint64_t>() + m);
+    const uint64_t WC = read_scalar<uint64_t>(c, c->rowC.as<uint64_t>() + m);
+    c->Aout = c->alloc.get(WA * 4 + 4, "b-role rows");
+    c->Ain = c->alloc.get(WA * 4 + 4, "a-role rows");
+    c->Cm = c->alloc.get(WC * 4 + 4, "c-role rows");
+    zero(c, c->Aout);
+    zero(c, c->Ain);
+    zero(c, c->Cm);
+    const BitRows rows = bitrows(c);
+    ktime(c, K_TRI_BITS, ranges.size(), [&] {
+      for (const auto& r : ranges)
+        launch_tri_bits(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
+                        c->inpos.as<uint32_t>(), c->s);
+    });
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
+      DevBuf cntA = c->alloc.get(m * 4 + 4, "a-role counts");
+      launch_row_counts(rows, m, cntA.as<uint32_t>(), nullptr, nullptr, c->s);
       DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(c->cntA.as<uint32_t>(), c->offA.as<uint64_t>(), m,
-                                         scratch.as<uint64_t>(), c->s);
+      exclusive_scan<uint32_t, uint64_t>(cntA.as<uint32_t>(), c->offA.as<uint64_t>(), m, scratch.as<uint64_t>(),
+                                         c->s);
       c->T = read_scalar<uint64_t>(c, c->offA.as<uint64_t>() + m);
     }
     if (c->T >= 0xFFFFFFFFull)
       throw Overflow("triangle count " + std::to_string(c->T) + " exceeds the 32-bit triangle id range");
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
-    DevBuf cur = c->alloc.get(m * 4 + 4, "triangle cursors");
-    zero(c, cur);
-    for (const auto& r : ranges)
-      launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second,
-                      c->offA.as<uint64_t>(), cur.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                      c->s);
-    segmented_sort_u64(c->triBC.as<uint64_t>(), c->offA.as<uint64_t>(), m, c->alloc, c->s);
+    ktime(c, K_TRI_EMIT, ranges.size(), [&] {
+      for (const auto& r : ranges)
+        launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
+                        c->offA.as<uint64_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->s);
+    });
   });
   c->has_tris = true;
   c->st.triangles = c->T;
+  c->st.incidence_entries = 3 * c->T;
 }
 
 void run_support(nuc34_ctx* c) {
+  const uint64_t T = c->T;
+  c->st.ms_support = timed(c, [&] {
+    c->sup = c->alloc.get(T * 4 + 4, "support");
+    c->supUp = c->alloc.get(T * 4 + 4, "support above");
+    c->own = c->alloc.get(T * 4 + 4, "owned K4 counts");
+    c->small = c->alloc.get(64, "counters");
+    zero(c, c->small);
+    ktime(c, K_SUPPORT, 1, [&] {
+      launch_support_bits(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), c->sup.as<int32_t>(),
+                          c->supUp.as<uint32_t>(), c->own.as<uint32_t>(),
+                          reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->s);
+    });
+    const uint64_t total = read_scalar<uint64_t>(c, c->small.as<char>() + 8);
+    if (total % 4) throw std::logic_error("support sum is not a multiple of 4");
+    c->Q = total / 4;
+  });
+  c->has_sup = true;
+  c->st.four_cliques = c->Q;
+}
+
+// Third-vertex-sorted edge->triangle lists, the on-demand peel's re-enumeration
+// structure (build_incidence_entries + sort_by_key, cuNucleus34_ondemand.cu:502-555).
+void run_incidence(nuc34_ctx* c) {
   const uint64_t m = c->m, T = c->T;
   c->st.ms_incidence = timed(c, [&] {
     DevBuf len = c->alloc.get(m * 4 + 4, "incidence lengths");
-    launch_inc_len(c->cntA.as<uint32_t>(), c->cntB.as<uint32_t>(), c->cntC.as<uint32_t>(), m, len.as<uint32_t>(),
-                   c->s);
+    {
+      DevBuf cA = c->alloc.get(m * 4 + 4, "role counts");
+      DevBuf cB = c->alloc.get(m * 4 + 4, "role counts");
+      DevBuf cC = c->alloc.get(m * 4 + 4, "role counts");
+      launch_row_counts(bitrows(c), m, cA.as<uint32_t>(), cB.as<uint32_t>(), cC.as<uint32_t>(), c->s);
+      launch_inc_len(cA.as<uint32_t>(), cB.as<uint32_t>(), cC.as<uint32_t>(), m, len.as<uint32_t>(), c->s);
+    }
     c->incO = c->alloc.get((m + 1) * 8, "incidence offsets");
     {
       DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
@@ -212,62 +346,46 @@ void run_support(nuc34_ctx* c) {
     c->third = c->alloc.get(3 * T * 4 + 4, "incidence third vertices");
     c->tri = c->alloc.get(3 * T * 4 + 4, "incidence triangles");
     launch_inc_split(inc.as<uint64_t>(), 3 * T, c->third.as<uint32_t>(), c->tri.as<uint32_t>(), c->s);
-  });
-  c->cntA.release();
-  c->cntB.release();
-  c->cntC.release();
-  c->st.incidence_entries = 3 * T;
-  c->st.ms_support = timed(c, [&] {
-    c->sup = c->alloc.get(T * 4 + 4, "support");
-    c->own = c->alloc.get(T * 4 + 4, "owned K4 counts");
     c->bigq = c->alloc.get(T * 4 + 4, "wave queue");
-    c->small = c->alloc.get(64, "counters");
-    zero(c, c->small);
-    TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
-              c->tri.as<uint32_t>()};
-    launch_support(v, c->sup.as<int32_t>(), c->own.as<uint32_t>(),
-                   reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->bigq.as<uint32_t>(),
-                   c->small.as<uint32_t>(), c->s);
-    const uint64_t total = read_scalar<uint64_t>(c, c->small.as<char>() + 8);
-    if (total % 4) throw std::logic_error("support sum is not a multiple of 4");
-    c->Q = total / 4;
   });
-  c->has_sup = true;
-  c->st.four_cliques = c->Q;
+  c->has_inc = true;
 }
 
-void run_k4_list(nuc34_ctx* c) {
+void run_k4_lists(nuc34_ctx* c) {
   const uint64_t T = c->T, Q = c->Q;
-  if (Q >= 0xFFFFFFFFull)
-    throw Overflow("four-clique count " + std::to_string(Q) + " exceeds the 32-bit K4 id range of the direct variant");
   c->st.ms_k4_list = timed(c, [&] {
-    c->k4off = c->alloc.get((T + 1) * 8, "K4 offsets");
-    {
-      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), c->k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
-                                         c->s);
-    }
-    const uint64_t q2 = read_scalar<uint64_t>(c, c->k4off.as<uint64_t>() + T);
-    if (q2 != Q) throw std::logic_error("owned K4 count disagrees with the support sum");
-    c->qud = c->alloc.get(Q * 16 + 16, "K4 list");
-    TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
-              c->tri.as<uint32_t>()};
-    launch_k4_emit(v, c->own.as<uint32_t>(), c->k4off.as<uint64_t>(), c->qud.as<uint32_t>(), c->bigq.as<uint32_t>(),
-                   c->small.as<uint32_t>(), c->s);
-    c->koff = c->alloc.get((T + 1) * 8, "triangle->K4 offsets");
+    c->koff = c->alloc.get((T + 1) * 8, "K4 list offsets");
     {
       DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
       exclusive_scan<int32_t, uint64_t>(c->sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
                                         c->s);
     }
-    c->incK = c->alloc.get(4 * Q * 4 + 4, "triangle->K4 incidence");
-    DevBuf cur = c->alloc.get(T * 4 + 4, "K4 cursors");
-    zero(c, cur);
-    launch_k4_incidence(c->qud.as<uint32_t>(), Q, c->koff.as<uint64_t>(), cur.as<uint32_t>(), c->incK.as<uint32_t>(),
-                        c->s);
+    c->lists = c->alloc.get(4 * Q * 12 + 12, "K4 partner lists");
+    DevBuf k4off = c->alloc.get((T + 1) * 8, "owned K4 offsets");
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
+                                         c->s);
+    }
+    if (read_scalar<uint64_t>(c, k4off.as<uint64_t>() + T) != Q)
+      throw std::logic_error("owned K4 count disagrees with the support sum");
+    DevBuf k4 = c->alloc.get(Q * 8 + 8, "owned K4s");
+    ktime(c, K_K4_OWNED, 1, [&] {
+      launch_k4_owned(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), k4off.as<uint64_t>(),
+                      k4.as<uint64_t>(), c->s);
+    });
+    ktime(c, K_K4_PLACE, 1, [&] {
+      launch_k4_place(k4.as<uint64_t>(), Q, c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->O.as<uint32_t>(),
+                      c->E.as<uint32_t>(), c->src.as<uint32_t>(), c->inpos.as<uint32_t>(), bitrows(c),
+                      c->offA.as<uint64_t>(), c->koff.as<uint64_t>(), c->supUp.as<uint32_t>(),
+                      c->lists.as<uint32_t>(), c->s);
+    });
   });
   c->has_k4 = true;
 }
+
+// Rounds enqueued per host check of the done flag (captured once as a graph).
+constexpr int kRoundsPerLaunch = 32;
 
 void run_peel(nuc34_ctx* c, int variant) {
   const uint64_t T = c->T;
@@ -282,70 +400,56 @@ void run_peel(nuc34_ctx* c, int variant) {
     NUC34_HIP(hipStreamSynchronize(c->s));
     return;
   }
-  const uint32_t T32 = static_cast<uint32_t>(T);
   DevBuf dd = c->alloc.get(T * 4, "peel supports");
-  DevBuf rnd = c->alloc.get(T * 4, "round stamps");
-  DevBuf alive = c->alloc.get(T * 4, "alive list");
-  DevBuf alive2 = c->alloc.get(T * 4, "alive list");
-  DevBuf front = c->alloc.get(T * 4, "frontier");
-  DevBuf next = c->alloc.get(T * 4, "frontier");
-  DevBuf pcb = c->alloc.get(sizeof(PeelCounters), "peel counters");
-  PeelCounters* pc = pcb.as<PeelCounters>();
-  TriView v{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->incO.as<uint64_t>(), c->third.as<uint32_t>(),
-            c->tri.as<uint32_t>()};
-  int32_t level = 0;
-  uint32_t round = 0;
-  uint64_t peeled = 0;
-  c->st.ms_peel = timed(c, [&] {
-    launch_fill_i32(c->kv.as<int32_t>(), T, -1, c->s);
-    launch_fill_u32(rnd.as<uint32_t>(), T, 0xFFFFFFFFu, c->s);
-    NUC34_HIP(hipMemcpyAsync(dd.as<void>(), c->sup.as<void>(), T * 4, hipMemcpyDeviceToDevice, c->s));
-    launch_iota(alive.as<uint32_t>(), T, c->s);
-    NUC34_HIP(hipMemsetAsync(pc, 0, sizeof(PeelCounters), c->s));
-    uint32_t nalive = T32, fcount = 0;
-    uint64_t last_compact = T;
-    while (peeled < T) {
-      if (fcount == 0) {
-        NUC34_HIP(hipMemsetAsync(&pc->minval, 0xFF, 4, c->s));
-        launch_alive_min(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), dd.as<int32_t>(), pc, c->s);
-        const uint32_t mn = read_scalar<uint32_t>(c, &pc->minval);
-        if (mn == 0xFFFFFFFFu) throw std::logic_error("peel: no alive triangle left but peeled < T");
-        level = std::max<int32_t>(level, static_cast<int32_t>(mn));
-        NUC34_HIP(hipMemsetAsync(&pc->front, 0, 4, c->s));
-        launch_collect(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), dd.as<int32_t>(), rnd.as<uint32_t>(), level,
-                       round, front.as<uint32_t>(), pc, c->s);
-        fcount = read_scalar<uint32_t>(c, &pc->front);
-        if (!fcount) throw std::logic_error("peel: empty frontier at a new level");
-      } else {
-        launch_mark(front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(), level, round, c->s);
-      }
-      NUC34_HIP(hipMemsetAsync(&pc->next, 0, 8, c->s));  // next, big
-      if (variant == NUC34_DIRECT) {
-        launch_debit_direct(c->koff.as<uint64_t>(), c->incK.as<uint32_t>(), c->qud.as<uint32_t>(),
-                            front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(), dd.as<int32_t>(), level,
-                            round, next.as<uint32_t>(), c->s);
-      } else {
-        launch_debit_ondemand(v, front.as<uint32_t>(), pc, c->kv.as<int32_t>(), rnd.as<uint32_t>(),
-                              dd.as<int32_t>(), level, round, next.as<uint32_t>(), c->bigq.as<uint32_t>(), c->s);
-      }
-      const uint32_t ncount = read_scalar<uint32_t>(c, &pc->next);
-      peeled += fcount;
-      ++round;
-      std::swap(front, next);
-      NUC34_HIP(hipMemcpyAsync(&pc->front, &pc->next, 4, hipMemcpyDeviceToDevice, c->s));
-      fcount = ncount;
-      if (peeled < T && (T - peeled) * 2 <= last_compact) {
-        NUC34_HIP(hipMemsetAsync(&pc->alive, 0, 4, c->s));
-        launch_compact_alive(alive.as<uint32_t>(), nalive, c->kv.as<int32_t>(), alive2.as<uint32_t>(), pc, c->s);
-        nalive = read_scalar<uint32_t>(c, &pc->alive);
-        std::swap(alive, alive2);
-        last_compact = nalive;
-      }
+  DevBuf st = c->alloc.get(T * 8, "peel state");
+  DevBuf b0 = c->alloc.get(T * 4, "frontier");
+  DevBuf b1 = c->alloc.get(T * 4, "frontier");
+  DevBuf a0 = c->alloc.get(T * 4, "alive list");
+  DevBuf a1 = c->alloc.get(T * 4, "alive list");
+  DevBuf ctlb = c->alloc.get(sizeof(PeelCtl), "peel control");
+  PeelBufs pb{dd.as<int32_t>(), st.as<uint64_t>(), b0.as<uint32_t>(), b1.as<uint32_t>(), a0.as<uint32_t>(),
+              a1.as<uint32_t>(), variant == NUC34_DIRECT ? nullptr : c->bigq.as<uint32_t>(), ctlb.as<PeelCtl>(), T};
+  const TriView v = triview(c);
+  auto enqueue_rounds = [&](int n) {
+    for (int r = 0; r < n; ++r) {
+      if (variant == NUC34_DIRECT) launch_peel_round_direct(pb, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s);
+      else launch_peel_round_ondemand(pb, v, c->s);
     }
+  };
+  PeelCtl h{};
+  c->st.ms_peel = timed(c, [&] {
+    launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {
+      enqueue_rounds(kRoundsPerLaunch);
+      if (hipStreamEndCapture(c->s, &graph) != hipSuccess ||
+          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        exec = nullptr;
+        (void)hipGetLastError();
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+    const uint64_t max_launches = 2 * T / kRoundsPerLaunch + 16;  // every round fixes >= 1 triangle
+    for (uint64_t l = 0;; ++l) {
+      ktime(c, K_PEEL, kRoundsPerLaunch, [&] {
+        if (exec) NUC34_HIP(hipGraphLaunch(exec, c->s));
+        else enqueue_rounds(kRoundsPerLaunch);
+      });
+      NUC34_HIP(hipMemcpyAsync(&h, ctlb.as<void>(), sizeof(h), hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+      if (h.done) break;
+      if (l > max_launches) throw std::logic_error("peel did not finish");
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    launch_peel_extract(pb, c->kv.as<int32_t>(), c->s);
   });
-  c->st.rounds = round;
-  c->st.k_max = level;
-  c->st.unpeeled = T - peeled;
+  if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
+  c->st.rounds = h.round;
+  c->st.k_max = h.level;
+  c->st.unpeeled = T - h.peeled;
 }
 
 // Host-side checks that keep every device index in bounds: monotone offsets,
@@ -377,8 +481,8 @@ std::string validate_csr(const uint32_t* O, const uint32_t* E, uint32_t n, bool 
 }
 
 uint64_t direct_bytes(const nuc34_ctx* c) {
-  // K4 list, triangle->K4 incidence, two offset arrays, scan scratch
-  return c->Q * 16 + c->Q * 16 + (c->T + 1) * 16 + (c->T / 1024 + 64) * 8;
+  // per-triangle K4 partner lists (4 entries of 3 ids per K4), offsets, scratch
+  return c->Q * 48 + (c->T + 1) * 8 + (c->T / 1024 + 64) * 8;
 }
 
 }  // namespace
@@ -467,6 +571,11 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->s) (void)hipStreamSynchronize(c->s);
   reset_stages(c, 0);
+  c->alloc.trim();
+  for (auto& k : c->kt) {
+    if (k.a) (void)hipEventDestroy(k.a);
+    if (k.b) (void)hipEventDestroy(k.b);
+  }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -604,6 +713,7 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
     return fail(c, NUC34_E_INVALID, "unknown variant");
   return guard(c, [&]() -> int {
     if (!c->has_tris) {
+      ktime_reset(c);
       reset_stages(c, 1);
       run_triangles(c, parts ? parts : 1);
     }
@@ -616,17 +726,27 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
     if (v == NUC34_AUTO) {
       size_t free_b = 0, total_b = 0;
       NUC34_HIP(hipMemGetInfo(&free_b, &total_b));
-      uint64_t room = static_cast<uint64_t>(0.8 * static_cast<double>(free_b));
+      uint64_t room = static_cast<uint64_t>(0.8 * static_cast<double>(free_b + c->alloc.cached));
       if (c->alloc.budget) room = std::min<uint64_t>(room, c->alloc.budget > c->alloc.used ? c->alloc.budget - c->alloc.used : 0);
-      v = (c->Q < 0xFFFFFFFFull && direct_bytes(c) + c->T * 28 <= room) ? NUC34_DIRECT : NUC34_ONDEMAND;
+      v = (direct_bytes(c) + c->T * 28 <= room) ? NUC34_DIRECT : NUC34_ONDEMAND;
     }
     reset_stages(c, 4);
-    c->st.ms_k4_list = 0;
-    if (v == NUC34_DIRECT && c->Q && !c->has_k4) run_k4_list(c);
+    if (c->Q) {
+      if (v == NUC34_DIRECT && !c->has_k4) {
+        c->st.ms_incidence = 0;
+        run_k4_lists(c);
+      }
+      if (v == NUC34_ONDEMAND && !c->has_inc) {
+        c->st.ms_k4_list = 0;
+        run_incidence(c);
+      }
+    }
     run_peel(c, v);
     c->st.variant = v;
     c->st.peak_bytes = c->alloc.peak;
-    c->st.ms_total = c->st.ms_triangles + c->st.ms_incidence + c->st.ms_support + c->st.ms_k4_list + c->st.ms_peel;
+    ktime_collect(c);
+    c->st.ms_total = c->st.ms_triangles + c->st.ms_support +
+                     (v == NUC34_DIRECT ? c->st.ms_k4_list : c->st.ms_incidence) + c->st.ms_peel;
     if (stats) *stats = c->st;
     return NUC34_OK;
   });
@@ -644,13 +764,38 @@ int nuc34_copy_k(nuc34_ctx* c, int32_t* k) {
 }
 
 int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t* quds) {
-  if (!c || !c->has_k4) return fail(c, NUC34_E_STATE, "K4 list not built (direct variant only)");
+  if (!c || !c->has_k4) return fail(c, NUC34_E_STATE, "K4 lists not built (direct variant only)");
   if (first > c->Q || count > c->Q - first) return fail(c, NUC34_E_INVALID, "K4 range out of bounds");
   return guard(c, [&]() -> int {
-    if (count) {
-      NUC34_HIP(hipMemcpyAsync(quds, c->qud.as<uint32_t>() + 4 * first, count * 16, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
+    if (!count) return NUC34_OK;
+    const uint64_t T = c->T;
+    DevBuf cnt = c->alloc.get(T * 4 + 4, "K4 counts");
+    DevBuf qoff = c->alloc.get((T + 1) * 8, "K4 offsets");
+    launch_k4_canon_count(c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), T, cnt.as<uint32_t>(), c->s);
+    {
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
+      exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), qoff.as<uint64_t>(), T, scratch.as<uint64_t>(), c->s);
     }
+    DevBuf out = c->alloc.get(count * 16, "K4 copy");
+    launch_k4_canon_fill(c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), T, qoff.as<uint64_t>(), first, count,
+                         out.as<uint32_t>(), c->s);
+    NUC34_HIP(hipMemcpyAsync(quds, out.as<void>(), count * 16, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    return NUC34_OK;
+  });
+}
+
+int nuc34_reset(nuc34_ctx* c) {
+  if (!c) return NUC34_E_INVALID;
+  return guard(c, [&]() -> int {
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    reset_stages(c, 1);
+    const nuc34_stats keep = c->st;
+    c->st = nuc34_stats{};
+    c->st.n = keep.n;
+    c->st.m = keep.m;
+    c->st.ms_orient = keep.ms_orient;
+    ktime_reset(c);
     return NUC34_OK;
   });
 }

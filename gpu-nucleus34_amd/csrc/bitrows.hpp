@@ -1,0 +1,68 @@
+// Bit-packed edge -> triangle incidence.
+//
+// For an oriented edge e with source u (out-list N+(u), in-list N-(u)):
+//   Aout row of e=(u,v): bit j set  <=>  triangle (u, v, N+(u)[j])   (e is its b-edge)
+//   Ain  row of e=(u,w): bit j set  <=>  triangle (u, N+(u)[j], w)   (e is its a-edge)
+//   Cm   row of e=(v,w): bit i set  <=>  triangle (N-(v)[i], v, w)   (e is its c-edge)
+// (vertices named by rank as in the reference's tri_t: a=(u,w), b=(u,v),
+// c=(v,w), u<v<w; nucleus/utils.cuh:93-102). Aout/Ain rows of e hold
+// ceil(d+(u)/32) words, Cm rows ceil(d-(u)/32) words, at per-edge word offsets.
+//
+// With these rows:
+//   * triangle index       = offA[a] + (bits of Ain[a] below v's position)
+//   * K4s of t=(u,v,w) above u: bits of (Aout|Ain)[a] & (Aout|Ain)[b]
+//     K4s of t below u:          bits of Cm[a] & Cm[b]
+//   * the three partner triangles of each K4 follow from the same prefix counts.
+#pragma once
+#include "common.hpp"
+
+namespace nuc34 {
+
+struct BitRows {
+  const uint64_t* rowA;  // m + 1 word offsets of the Aout / Ain rows
+  const uint64_t* rowC;  // m + 1 word offsets of the Cm rows
+  uint32_t* Aout;
+  uint32_t* Ain;
+  uint32_t* Cm;
+};
+
+// Set bits of row[0..) strictly below bit position j.
+__device__ __forceinline__ uint32_t bits_below(const uint32_t* row, uint32_t j) {
+  uint32_t n = 0;
+  const uint32_t full = j >> 5;
+  for (uint32_t i = 0; i < full; ++i) n += static_cast<uint32_t>(__popc(row[i]));
+  if (j & 31) n += static_cast<uint32_t>(__popc(row[full] & ((1u << (j & 31)) - 1u)));
+  return n;
+}
+
+// Set bits of (x & y) strictly below bit position j.
+__device__ __forceinline__ uint32_t and_bits_below(const uint32_t* x, const uint32_t* y, uint32_t j) {
+  uint32_t n = 0;
+  const uint32_t full = j >> 5;
+  for (uint32_t i = 0; i < full; ++i) n += static_cast<uint32_t>(__popc(x[i] & y[i]));
+  if (j & 31) n += static_cast<uint32_t>(__popc(x[full] & y[full] & ((1u << (j & 31)) - 1u)));
+  return n;
+}
+
+// Set bits of (x1|x2) & (y1|y2) strictly below bit position j.
+__device__ __forceinline__ uint32_t sym_bits_below(const uint32_t* x1, const uint32_t* x2, const uint32_t* y1,
+                                                   const uint32_t* y2, uint32_t j) {
+  uint32_t n = 0;
+  const uint32_t full = j >> 5;
+  for (uint32_t i = 0; i < full; ++i) n += static_cast<uint32_t>(__popc((x1[i] | x2[i]) & (y1[i] | y2[i])));
+  if (j & 31)
+    n += static_cast<uint32_t>(__popc((x1[full] | x2[full]) & (y1[full] | y2[full]) & ((1u << (j & 31)) - 1u)));
+  return n;
+}
+
+// In-CSR (transpose of the oriented graph) with lists sorted by source id,
+// plus inpos[e] = position of edge e in its target's in-list.
+void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint32_t n, uint64_t m, uint32_t* IO,
+                  uint32_t* IE, uint32_t* IEe, uint32_t* inpos, class Allocator& alloc, hipStream_t s);
+// Per-edge row lengths in words: wa[e] = ceil(d+(src e)/32), wc[e] = ceil(d-(src e)/32).
+void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src, uint64_t m, uint32_t* wa,
+                      uint32_t* wc, hipStream_t s);
+// Role counts from the rows: cntA = |Ain|, cntB = |Aout|, cntC = |Cm| per edge.
+void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s);
+
+}  // namespace nuc34

@@ -173,224 +173,415 @@ __global__ __launch_bounds__(kBlock) void k_support_big(TriView v, const uint32_
   }
 }
 
-// ---- K4 list (direct) -------------------------------------------------------
+// ---- support and K4 lists from the bit rows --------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_k4_emit_small(TriView v, const uint32_t* __restrict__ own,
-                                                          const uint64_t* __restrict__ k4off,
-                                                          uint32_t* __restrict__ qud) {
+__global__ __launch_bounds__(kBlock) void k_support_bits(const uint32_t* __restrict__ triA,
+                                                         const uint64_t* __restrict__ triBC, uint64_t T, BitRows r,
+                                                         int32_t* __restrict__ sup, uint32_t* __restrict__ sup_up,
+                                                         uint32_t* __restrict__ own, uint64_t* __restrict__ total) {
+  uint64_t acc = 0;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < v.T; t += stride) {
-    if (!own[t]) continue;
-    const TriEdges e = tri_edges(v, t);
-    if (tri_len(v, e) > kSmallL) continue;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T; t += stride) {
+    const uint32_t a = triA[t];
+    const uint32_t b = static_cast<uint32_t>(triBC[t] >> 32);
+    const uint64_t ra = r.rowA[a], rb = r.rowA[b];
+    const uint32_t W = static_cast<uint32_t>(r.rowA[a + 1] - ra);
+    uint32_t up = 0, ow = 0, down = 0;
+    for (uint32_t i = 0; i < W; ++i) {
+      const uint32_t oa = r.Aout[ra + i], ob = r.Aout[rb + i];
+      up += static_cast<uint32_t>(__popc((oa | r.Ain[ra + i]) & (ob | r.Ain[rb + i])));
+      ow += static_cast<uint32_t>(__popc(oa & ob));
+    }
+    const uint64_t ca = r.rowC[a], cb = r.rowC[b];
+    const uint32_t WC = static_cast<uint32_t>(r.rowC[a + 1] - ca);
+    for (uint32_t i = 0; i < WC; ++i) down += static_cast<uint32_t>(__popc(r.Cm[ca + i] & r.Cm[cb + i]));
+    sup[t] = static_cast<int32_t>(up + down);
+    sup_up[t] = up;
+    own[t] = ow;
+    acc += up + down;
+  }
+  acc = wave_sum_u64(acc);
+  if (lane_id() == 0 && acc) atomic_add_u64(total, acc);
+}
+
+__device__ __forceinline__ void put3(uint32_t* __restrict__ lists, uint64_t slot, uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t* q = lists + 3 * slot;
+  q[0] = x;
+  q[1] = y;
+  q[2] = z;
+}
+
+// Direct K4 lists, step 1: each triangle lists the K4s it owns (fourth vertex
+// above all three of its vertices) as (t << 32 | position of x in N+(u)).
+__global__ __launch_bounds__(kBlock) void k_k4_owned(const uint32_t* __restrict__ triA,
+                                                     const uint64_t* __restrict__ triBC, uint64_t T, BitRows r,
+                                                     const uint64_t* __restrict__ k4off, uint64_t* __restrict__ k4) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T; t += stride) {
+    const uint32_t a = triA[t];
+    const uint32_t b = static_cast<uint32_t>(triBC[t] >> 32);
+    const uint32_t* Aa = r.Aout + r.rowA[a];
+    const uint32_t* Ab = r.Aout + r.rowA[b];
+    const uint32_t W = static_cast<uint32_t>(r.rowA[a + 1] - r.rowA[a]);
     uint64_t pos = k4off[t];
-    k4_thread(v, e, [&](uint32_t pb, uint32_t pa, uint32_t pc) {
-      if (!owns_k4(v, pc, e.c)) return;
-      uint32_t* q = qud + 4 * pos;
-      q[0] = pb;
-      q[1] = static_cast<uint32_t>(t);
-      q[2] = pa;
-      q[3] = pc;
-      ++pos;
-    });
+    for (uint32_t wi = 0; wi < W; ++wi) {
+      uint32_t bits = Aa[wi] & Ab[wi];
+      while (bits) {
+        const uint32_t jx = wi * 32 + static_cast<uint32_t>(__builtin_ctz(bits));
+        bits &= bits - 1;
+        k4[pos++] = (t << 32) | jx;
+      }
+    }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_k4_emit_big(TriView v, const uint32_t* __restrict__ bigq,
-                                                        const uint32_t* __restrict__ bigcount,
-                                                        const uint32_t* __restrict__ own,
-                                                        const uint64_t* __restrict__ k4off,
-                                                        uint32_t* __restrict__ qud) {
-  __shared__ uint32_t s1[kWPB][kWave], s2[kWPB][kWave];
-  const unsigned w = threadIdx.x / kWave;
-  const uint32_t n = *bigcount;
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t q = blockIdx.x * kWPB + w; q < n; q += nw) {
-    const uint32_t t = bigq[q];
-    if (!own[t]) continue;
-    const TriEdges e = tri_edges(v, t);
-    uint64_t pos = k4off[t];
-    k4_wave(v, e, s1[w], s2[w], [&](bool match, uint64_t i0, uint64_t i1, uint64_t i2) {
-      uint32_t pc = 0;
-      bool o = false;
-      if (match) {
-        pc = v.tri[i2];
-        o = owns_k4(v, pc, e.c);
-      }
-      const uint64_t mask = __ballot(o);
-      if (o) {
-        uint32_t* out = qud + 4 * (pos + lanes_below(mask));
-        out[0] = v.tri[i0];
-        out[1] = t;
-        out[2] = v.tri[i1];
-        out[3] = pc;
-      }
-      pos += static_cast<uint64_t>(__popcll(mask));
-    });
-  }
-}
-
-__global__ void k_k4_incidence(const uint32_t* __restrict__ qud, uint64_t Q, const uint64_t* __restrict__ koff,
-                               uint32_t* __restrict__ cur, uint32_t* __restrict__ incK) {
+// Direct K4 lists, step 2: one thread per K4 {u < v < w < x} (t = T(u,v,w)):
+// the other three triangle ids from prefix counts of the bit rows, then one
+// entry (the other three ids) in the list of each of the four triangles at a
+// position fixed by prefix counts: up-K4s (fourth vertex above the triangle's
+// lowest vertex) first, in fourth-vertex order, then down-K4s.
+__global__ __launch_bounds__(kBlock) void k_k4_place(const uint64_t* __restrict__ k4, uint64_t Q,
+                                                     const uint32_t* __restrict__ triA,
+                                                     const uint64_t* __restrict__ triBC,
+                                                     const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
+                                                     const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ inpos, BitRows r,
+                                                     const uint64_t* __restrict__ offA,
+                                                     const uint64_t* __restrict__ koff,
+                                                     const uint32_t* __restrict__ sup_up,
+                                                     uint32_t* __restrict__ lists) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < Q; q += stride) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const uint32_t t = qud[4 * q + s];
-      incK[koff[t] + atomic_add_u32(&cur[t], 1u)] = static_cast<uint32_t>(q);
+    const uint64_t key = k4[q];
+    const uint32_t t = static_cast<uint32_t>(key >> 32);
+    const uint32_t jx = static_cast<uint32_t>(key);
+    const uint32_t a = triA[t];
+    const uint64_t bc = triBC[t];
+    const uint32_t b = static_cast<uint32_t>(bc >> 32);
+    const uint32_t c = static_cast<uint32_t>(bc);
+    const uint32_t ou = O[src[b]];
+    const uint32_t jv = b - ou, jw = a - ou;
+    const uint32_t* Aa = r.Aout + r.rowA[a];
+    const uint32_t* Ab = r.Aout + r.rowA[b];
+    const uint32_t* Ia = r.Ain + r.rowA[a];
+    const uint32_t* Ib = r.Ain + r.rowA[b];
+    const uint32_t ex = ou + jx;
+    const uint32_t* Ax = r.Aout + r.rowA[ex];
+    const uint32_t* Ix = r.Ain + r.rowA[ex];
+    const uint32_t t1 = static_cast<uint32_t>(offA[ex] + bits_below(Ix, jv));  // T(u,v,x)
+    const uint32_t t2 = static_cast<uint32_t>(offA[ex] + bits_below(Ix, jw));  // T(u,w,x)
+    const uint32_t c1 = static_cast<uint32_t>(triBC[t1]);                        // idx(v,x)
+    const uint32_t ov = O[E[b]];
+    const uint32_t t3 = static_cast<uint32_t>(offA[c1] + bits_below(r.Ain + r.rowA[c1], c - ov));  // T(v,w,x)
+    put3(lists, koff[t] + sym_bits_below(Aa, Ia, Ab, Ib, jx), t1, t2, t3);
+    put3(lists, koff[t1] + sym_bits_below(Ab, Ib, Ax, Ix, jw), t, t2, t3);
+    put3(lists, koff[t2] + sym_bits_below(Aa, Ia, Ax, Ix, jv), t, t1, t3);
+    put3(lists, koff[t3] + sup_up[t3] + and_bits_below(r.Cm + r.rowC[c], r.Cm + r.rowC[c1], inpos[b]), t, t1, t2);
+  }
+}
+
+__global__ void k_k4_canon_count(const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists, uint64_t T,
+                                 uint32_t* __restrict__ cnt) {
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t n = 0;
+    for (uint64_t j = koff[t]; j < koff[t + 1]; ++j) {
+      const uint32_t* q = lists + 3 * j;
+      n += (t < q[0] && t < q[1] && t < q[2]) ? 1u : 0u;
+    }
+    cnt[t] = n;
+  }
+}
+
+__global__ void k_k4_canon_fill(const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists, uint64_t T,
+                                const uint64_t* __restrict__ qoff, uint64_t first, uint64_t count,
+                                uint32_t* __restrict__ qud) {
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint64_t pos = qoff[t];
+    if (qoff[t + 1] <= first || pos >= first + count) continue;
+    for (uint64_t j = koff[t]; j < koff[t + 1]; ++j) {
+      const uint32_t* q = lists + 3 * j;
+      if (t < q[0] && t < q[1] && t < q[2]) {
+        const uint64_t at = pos++;
+        if (at < first || at >= first + count) continue;
+        uint32_t* o = qud + 4 * (at - first);
+        o[0] = static_cast<uint32_t>(t);
+        o[1] = q[0];
+        o[2] = q[1];
+        o[3] = q[2];
+      }
     }
   }
 }
 
-// ---- peel -------------------------------------------------------------------
+// ---- peel ------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_alive_min(const uint32_t* __restrict__ alive, uint32_t nalive,
-                                                      const int32_t* __restrict__ kv, const int32_t* __restrict__ dd,
-                                                      PeelCounters* __restrict__ pc) {
-  uint32_t mn = kNone;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nalive; i += gridDim.x * blockDim.x) {
-    const uint32_t t = alive[i];
-    if (kv[t] == -1) {
-      const int32_t d = dd[t];
-      mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
-    }
+constexpr int32_t kDead = 0x7FFFFFFF;
+constexpr uint64_t kAlive = ~0ull;
+
+// Level in force for this round: when the frontier ran empty, the new level is
+// max(level, min alive support) from k_peel_level (committed by k_peel_finish).
+__device__ __forceinline__ int32_t round_level(const PeelCtl* c) {
+  if (!c->need_level) return c->level;
+  const uint32_t m = c->minval;
+  return (m != 0xFFFFFFFFu && static_cast<int32_t>(m) > c->level) ? static_cast<int32_t>(m) : c->level;
+}
+
+__global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t* __restrict__ sup) {
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < p.T;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    p.dd[t] = sup[t];
+    p.st[t] = kAlive;
+    p.alive0[t] = static_cast<uint32_t>(t);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    PeelCtl* c = p.ctl;
+    c->count[0] = c->count[1] = 0;
+    c->parity = 0;
+    c->need_level = 1;
+    c->minval = 0xFFFFFFFFu;
+    c->level = 0;
+    c->round = 0;
+    c->done = 0;
+    c->peeled = 0;
+    c->big = 0;
+    c->error = 0;
+    c->nalive[0] = static_cast<uint32_t>(p.T);
+    c->nalive[1] = 0;
+    c->apar = 0;
+  }
+}
+
+// Level selection, step 1 (only when the frontier ran empty): min support over
+// the alive list; one atomic per workgroup.
+__global__ __launch_bounds__(kBlock) void k_peel_level(PeelBufs p) {
+  const PeelCtl* c = p.ctl;
+  if (c->done || !c->need_level) return;
+  __shared__ uint32_t wmin[kWPB];
+  uint32_t mn = 0xFFFFFFFFu;
+  const uint32_t* alive = c->apar ? p.alive1 : p.alive0;
+  const uint32_t na = c->nalive[c->apar];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const int32_t d = p.dd[alive[i]];
+    if (d != kDead) mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mn = min(mn, static_cast<uint32_t>(__shfl_xor(mn, off)));
-  if (lane_id() == 0 && mn != kNone) atomic_min_u32(&pc->minval, mn);
+  if (lane_id() == 0) wmin[threadIdx.x / kWave] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWPB; ++i) mn = min(mn, wmin[i]);
+    if (mn != 0xFFFFFFFFu) atomic_min_u32(&p.ctl->minval, mn);
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_collect(const uint32_t* __restrict__ alive, uint32_t nalive,
-                                                    int32_t* __restrict__ kv, const int32_t* __restrict__ dd,
-                                                    uint32_t* __restrict__ rnd, int32_t level, uint32_t round,
-                                                    uint32_t* __restrict__ front, PeelCounters* __restrict__ pc) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nalive; i += gridDim.x * blockDim.x) {
-    const uint32_t t = alive[i];
-    const bool die = kv[t] == -1 && dd[t] <= level;
-    bool got;
-    const uint32_t slot = wave_append(&pc->front, die, &got);
-    if (die) {
-      kv[t] = level;
-      rnd[t] = round;
-      front[slot] = t;
+// Level selection, step 2: every alive triangle with support <= level dies this
+// round (k = level); the others move to the other alive buffer. Each thread
+// takes kCollectItems list entries per tile, so one pair of atomics reserves
+// the output slots of a whole 4096-entry tile.
+constexpr int kCollectItems = 16;
+
+__global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p) {
+  PeelCtl* c = p.ctl;
+  if (c->done || !c->need_level) return;
+  __shared__ uint32_t wd[kWPB], wk[kWPB], base[2];
+  const int32_t level = round_level(c);
+  const uint32_t round = c->round;
+  const uint32_t par = c->parity;
+  const uint32_t ap = c->apar;
+  const uint32_t* alive = ap ? p.alive1 : p.alive0;
+  uint32_t* keep = ap ? p.alive0 : p.alive1;
+  const uint32_t na = c->nalive[ap];
+  uint32_t* cur = par ? p.buf1 : p.buf0;
+  const unsigned w = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  constexpr uint32_t tile = kBlock * kCollectItems;
+  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * tile; b0 < na; b0 += static_cast<uint64_t>(gridDim.x) * tile) {
+    uint32_t tv[kCollectItems];
+    uint32_t die = 0, stay = 0;  // bit j: item j
+#pragma unroll
+    for (int j = 0; j < kCollectItems; ++j) {
+      const uint64_t i = b0 + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
+      tv[j] = 0;
+      if (i < na) {
+        const uint32_t t = alive[i];
+        const int32_t d = p.dd[t];
+        tv[j] = t;
+        if (d <= level) die |= 1u << j;
+        else if (d != kDead) stay |= 1u << j;
+      }
+    }
+    const uint32_t nd = static_cast<uint32_t>(__popc(die)), nk = static_cast<uint32_t>(__popc(stay));
+    const uint32_t id = wave_inclusive_sum(nd), ik = wave_inclusive_sum(nk);
+    if (lane == kWave - 1) {
+      wd[w] = id;
+      wk[w] = ik;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t td = 0, tk = 0;
+      for (int q = 0; q < kWPB; ++q) {
+        const uint32_t x = wd[q], y = wk[q];
+        wd[q] = td;
+        wk[q] = tk;
+        td += x;
+        tk += y;
+      }
+      base[0] = td ? atomic_add_u32(&c->count[par], td) : 0;
+      base[1] = tk ? atomic_add_u32(&c->nalive[ap ^ 1u], tk) : 0;
+    }
+    __syncthreads();
+    uint32_t od = base[0] + wd[w] + id - nd, ok = base[1] + wk[w] + ik - nk;
+#pragma unroll
+    for (int j = 0; j < kCollectItems; ++j) {
+      if (die & (1u << j)) {
+        p.st[tv[j]] = (static_cast<uint64_t>(round) << 32) | static_cast<uint32_t>(level);
+        p.dd[tv[j]] = kDead;
+        cur[od++] = tv[j];
+      } else if (stay & (1u << j)) {
+        keep[ok++] = tv[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Owner test and debit for one K4 of dying triangle t with partners q[0..2].
+// A partner that is alive, or was pushed this round (dies next round), counts
+// as surviving for ownership; an earlier death means the K4 is gone; a
+// same-round death with a lower index owns it. A survivor falling from
+// level+1 to level is pushed: its k is fixed now and it dies next round.
+__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
+                                           uint32_t round, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+  uint64_t sq[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    sq[s] = p.st[q[s]];
+    const uint32_t r = static_cast<uint32_t>(sq[s] >> 32);
+    if (r < round) return;
+    if (r == round && q[s] < t) return;
+  }
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (sq[s] != kAlive) continue;
+    const int32_t old = atomic_sub_i32(&p.dd[q[s]], 1);
+    if (old == level + 1) {
+      p.st[q[s]] = (static_cast<uint64_t>(round + 1) << 32) | static_cast<uint32_t>(level);
+      __hip_atomic_store(&p.dd[q[s]], kDead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nxt[atomic_add_u32(ncount, 1u)] = q[s];
     }
   }
 }
 
-__global__ void k_mark(const uint32_t* __restrict__ front, const PeelCounters* __restrict__ pc,
-                       int32_t* __restrict__ kv, uint32_t* __restrict__ rnd, int32_t level, uint32_t round) {
-  const uint32_t n = pc->front;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = front[i];
-    kv[t] = level;
-    rnd[t] = round;
+// End of a round, one workgroup: commit the level, flip the buffers, count the
+// fixed triangles; an empty next frontier triggers level selection.
+__global__ void k_peel_finish(PeelBufs p) {
+  if (threadIdx.x != 0) return;
+  PeelCtl* c = p.ctl;
+  if (c->done) return;
+  if (c->need_level) {
+    c->level = round_level(c);
+    if (c->minval == 0xFFFFFFFFu || c->count[c->parity] == 0) {
+      c->error = c->minval == 0xFFFFFFFFu ? 1u : 2u;  // nothing alive / nothing collected
+      c->done = 1;
+      return;
+    }
+    c->minval = 0xFFFFFFFFu;
+    c->nalive[c->apar] = 0;
+    c->apar ^= 1u;
+  }
+  const uint32_t par = c->parity;
+  c->peeled += c->count[par];
+  c->count[par] = 0;
+  c->parity = par ^ 1u;
+  c->round += 1;
+  c->big = 0;
+  c->need_level = c->count[par ^ 1u] == 0 ? 1u : 0u;
+  if (c->peeled >= p.T) c->done = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, const uint64_t* __restrict__ koff,
+                                                             const uint32_t* __restrict__ lists) {
+  const PeelCtl* c = p.ctl;
+  if (c->done) return;
+  const int32_t level = round_level(c);
+  const uint32_t round = c->round;
+  const uint32_t par = c->parity;
+  const uint32_t* cur = par ? p.buf1 : p.buf0;
+  uint32_t* nxt = par ? p.buf0 : p.buf1;
+  uint32_t* ncount = &p.ctl->count[par ^ 1u];
+  const uint32_t n = c->count[par];
+  const unsigned w = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  const uint32_t nw = gridDim.x * kWPB;
+  for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
+    const uint32_t t = cur[i];
+    const uint64_t b = koff[t], e = koff[t + 1];
+    for (uint64_t j = b + lane; j < e; j += kWave) {
+      const uint32_t* q = lists + 3 * j;
+      const uint32_t qq[3] = {q[0], q[1], q[2]};
+      destroy_k4(t, qq, p, level, round, nxt, ncount);
+    }
   }
 }
 
-// Owner test and debit for one K4 of dying triangle t with partners p[0..2].
-__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t p[3], const int32_t* __restrict__ kv,
-                                           const uint32_t* __restrict__ rnd, int32_t* __restrict__ dd,
-                                           int32_t level, uint32_t round, uint32_t* __restrict__ next,
-                                           PeelCounters* __restrict__ pc) {
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    if (kv[p[s]] == -1) continue;     // still alive
-    if (rnd[p[s]] < round) return;    // destroyed in an earlier round
-    if (p[s] < t) return;             // same round: the lowest index owns it
-  }
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    if (kv[p[s]] != -1) continue;
-    const int32_t old = atomic_sub_i32(&dd[p[s]], 1);
-    if (old == level + 1) next[atomic_add_u32(&pc->next, 1u)] = p[s];
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_debit_od_small(TriView v, const uint32_t* __restrict__ front,
-                                                           PeelCounters* __restrict__ pc,
-                                                           const int32_t* __restrict__ kv,
-                                                           const uint32_t* __restrict__ rnd, int32_t* __restrict__ dd,
-                                                           int32_t level, uint32_t round, uint32_t* __restrict__ next,
-                                                           uint32_t* __restrict__ bigq) {
-  const uint32_t n = pc->front;
+__global__ __launch_bounds__(kBlock) void k_peel_debit_od_small(PeelBufs p, TriView v) {
+  PeelCtl* c = p.ctl;
+  if (c->done) return;
+  const int32_t level = round_level(c);
+  const uint32_t round = c->round;
+  const uint32_t par = c->parity;
+  const uint32_t* cur = par ? p.buf1 : p.buf0;
+  uint32_t* nxt = par ? p.buf0 : p.buf1;
+  uint32_t* ncount = &c->count[par ^ 1u];
+  const uint32_t n = c->count[par];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = front[i];
+    const uint32_t t = cur[i];
     const TriEdges e = tri_edges(v, t);
     const bool big = tri_len(v, e) > kSmallL;
     bool got;
-    const uint32_t slot = wave_append(&pc->big, big, &got);
+    const uint32_t slot = wave_append(&c->big, big, &got);
     if (big) {
-      bigq[slot] = t;
+      p.bigq[slot] = t;
       continue;
     }
     k4_thread(v, e, [&](uint32_t pb, uint32_t pa, uint32_t pcc) {
-      const uint32_t p[3] = {pb, pa, pcc};
-      destroy_k4(t, p, kv, rnd, dd, level, round, next, pc);
+      const uint32_t qq[3] = {pb, pa, pcc};
+      destroy_k4(t, qq, p, level, round, nxt, ncount);
     });
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_debit_od_big(TriView v, const uint32_t* __restrict__ bigq,
-                                                         PeelCounters* __restrict__ pc,
-                                                         const int32_t* __restrict__ kv,
-                                                         const uint32_t* __restrict__ rnd, int32_t* __restrict__ dd,
-                                                         int32_t level, uint32_t round,
-                                                         uint32_t* __restrict__ next) {
+__global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, TriView v) {
   __shared__ uint32_t s1[kWPB][kWave], s2[kWPB][kWave];
+  PeelCtl* c = p.ctl;
+  if (c->done) return;
+  const int32_t level = round_level(c);
+  const uint32_t round = c->round;
+  const uint32_t par = c->parity;
+  uint32_t* nxt = par ? p.buf0 : p.buf1;
+  uint32_t* ncount = &c->count[par ^ 1u];
+  const uint32_t n = c->big;
   const unsigned w = threadIdx.x / kWave;
-  const uint32_t n = pc->big;
   const uint32_t nw = gridDim.x * kWPB;
   for (uint32_t q = blockIdx.x * kWPB + w; q < n; q += nw) {
-    const uint32_t t = bigq[q];
+    const uint32_t t = p.bigq[q];
     const TriEdges e = tri_edges(v, t);
     k4_wave(v, e, s1[w], s2[w], [&](bool match, uint64_t i0, uint64_t i1, uint64_t i2) {
       if (match) {
-        const uint32_t p[3] = {v.tri[i0], v.tri[i1], v.tri[i2]};
-        destroy_k4(t, p, kv, rnd, dd, level, round, next, pc);
+        const uint32_t qq[3] = {v.tri[i0], v.tri[i1], v.tri[i2]};
+        destroy_k4(t, qq, p, level, round, nxt, ncount);
       }
     });
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_debit_direct(const uint64_t* __restrict__ koff,
-                                                         const uint32_t* __restrict__ incK,
-                                                         const uint32_t* __restrict__ qud,
-                                                         const uint32_t* __restrict__ front,
-                                                         PeelCounters* __restrict__ pc,
-                                                         const int32_t* __restrict__ kv,
-                                                         const uint32_t* __restrict__ rnd, int32_t* __restrict__ dd,
-                                                         int32_t level, uint32_t round,
-                                                         uint32_t* __restrict__ next) {
-  const unsigned w = threadIdx.x / kWave;
-  const unsigned lane = lane_id();
-  const uint32_t n = pc->front;
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
-    const uint32_t t = front[i];
-    const uint64_t b = koff[t], e = koff[t + 1];
-    for (uint64_t j = b + lane; j < e; j += kWave) {
-      const uint32_t q = incK[j];
-      const uint4 m = *reinterpret_cast<const uint4*>(qud + 4ull * q);
-      uint32_t p[3];
-      int k = 0;
-      if (m.x != t) p[k++] = m.x;
-      if (m.y != t) p[k++] = m.y;
-      if (m.z != t) p[k++] = m.z;
-      if (m.w != t && k < 3) p[k++] = m.w;
-      destroy_k4(t, p, kv, rnd, dd, level, round, next, pc);
-    }
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_compact(const uint32_t* __restrict__ alive, uint32_t nalive,
-                                                    const int32_t* __restrict__ kv, uint32_t* __restrict__ out,
-                                                    PeelCounters* __restrict__ pc) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nalive; i += gridDim.x * blockDim.x) {
-    const uint32_t t = alive[i];
-    const bool keep = kv[t] == -1;
-    bool got;
-    const uint32_t slot = wave_append(&pc->alive, keep, &got);
-    if (keep) out[slot] = t;
+__global__ void k_peel_extract(PeelBufs p, int32_t* __restrict__ kv) {
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < p.T;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t s = p.st[t];
+    kv[t] = s == kAlive ? -1 : static_cast<int32_t>(static_cast<uint32_t>(s));
   }
 }
 
@@ -408,69 +599,72 @@ void launch_support(const TriView& v, int32_t* support, uint32_t* owned, uint64_
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_emit(const TriView& v, const uint32_t* owned, const uint64_t* k4off, uint32_t* qud,
-                    const uint32_t* bigq, const uint32_t* bigcount, hipStream_t s) {
-  if (!v.T) return;
-  hipLaunchKernelGGL(k_k4_emit_small, dim3(grid_for(v.T, kBlock, 1u << 14)), dim3(kBlock), 0, s, v, owned, k4off,
-                     qud);
-  NUC34_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_k4_emit_big, dim3(8192), dim3(kBlock), 0, s, v, bigq, bigcount, owned, k4off, qud);
+void launch_support_bits(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows, int32_t* sup,
+                         uint32_t* sup_up, uint32_t* own, uint64_t* total, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(T, kBlock, 1u << 14)), dim3(kBlock), 0, s, triA, triBC, T, rows,
+                     sup, sup_up, own, total);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_incidence(const uint32_t* qud, uint64_t Q, const uint64_t* koff, uint32_t* cur, uint32_t* incK,
-                         hipStream_t s) {
+void launch_k4_owned(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows,
+                     const uint64_t* k4off, uint64_t* k4, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_k4_owned, dim3(grid_for(T, kBlock, 1u << 14)), dim3(kBlock), 0, s, triA, triBC, T, rows, k4off,
+                     k4);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_k4_place(const uint64_t* k4, uint64_t Q, const uint32_t* triA, const uint64_t* triBC, const uint32_t* O,
+                     const uint32_t* E, const uint32_t* src, const uint32_t* inpos, const BitRows& rows,
+                     const uint64_t* offA, const uint64_t* koff, const uint32_t* sup_up, uint32_t* lists,
+                     hipStream_t s) {
   if (!Q) return;
-  hipLaunchKernelGGL(k_k4_incidence, dim3(grid_for(Q, 256, 1u << 16)), dim3(256), 0, s, qud, Q, koff, cur, incK);
+  hipLaunchKernelGGL(k_k4_place, dim3(grid_for(Q, kBlock, 1u << 16)), dim3(kBlock), 0, s, k4, Q, triA, triBC, O, E,
+                     src, inpos, rows, offA, koff, sup_up, lists);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_alive_min(const uint32_t* alive, uint32_t nalive, const int32_t* kv, const int32_t* dd,
-                      PeelCounters* pc, hipStream_t s) {
-  if (!nalive) return;
-  hipLaunchKernelGGL(k_alive_min, dim3(grid_for(nalive, kBlock, 4096)), dim3(kBlock), 0, s, alive, nalive, kv, dd,
-                     pc);
+void launch_k4_canon_count(const uint64_t* koff, const uint32_t* lists, uint64_t T, uint32_t* cnt, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_k4_canon_count, dim3(grid_for(T, 256, 1u << 14)), dim3(256), 0, s, koff, lists, T, cnt);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_collect(const uint32_t* alive, uint32_t nalive, int32_t* kv, const int32_t* dd, uint32_t* rnd,
-                    int32_t level, uint32_t round, uint32_t* front, PeelCounters* pc, hipStream_t s) {
-  if (!nalive) return;
-  hipLaunchKernelGGL(k_collect, dim3(grid_for(nalive, kBlock, 4096)), dim3(kBlock), 0, s, alive, nalive, kv, dd,
-                     rnd, level, round, front, pc);
+void launch_k4_canon_fill(const uint64_t* koff, const uint32_t* lists, uint64_t T, const uint64_t* qoff,
+                          uint64_t first, uint64_t count, uint32_t* qud, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_k4_canon_fill, dim3(grid_for(T, 256, 1u << 14)), dim3(256), 0, s, koff, lists, T, qoff, first,
+                     count, qud);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_mark(const uint32_t* front, const PeelCounters* pc, int32_t* kv, uint32_t* rnd, int32_t level,
-                 uint32_t round, hipStream_t s) {
-  hipLaunchKernelGGL(k_mark, dim3(kRoundGrid), dim3(kBlock), 0, s, front, pc, kv, rnd, level, round);
+constexpr unsigned kScanGrid = 512;
+
+void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel_init, dim3(grid_for(p.T ? p.T : 1, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_debit_ondemand(const TriView& v, const uint32_t* front, PeelCounters* pc, const int32_t* kv,
-                           const uint32_t* rnd, int32_t* dd, int32_t level, uint32_t round, uint32_t* next,
-                           uint32_t* bigq, hipStream_t s) {
-  hipLaunchKernelGGL(k_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, v, front, pc, kv, rnd, dd, level,
-                     round, next, bigq);
-  NUC34_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, v, bigq, pc, kv, rnd, dd, level, round,
-                     next);
+void launch_peel_round_direct(const PeelBufs& p, const uint64_t* koff, const uint32_t* lists, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel_level, dim3(kScanGrid), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, koff, lists);
+  hipLaunchKernelGGL(k_peel_finish, dim3(1), dim3(kWave), 0, s, p);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_debit_direct(const uint64_t* koff, const uint32_t* incK, const uint32_t* qud, const uint32_t* front,
-                         PeelCounters* pc, const int32_t* kv, const uint32_t* rnd, int32_t* dd, int32_t level,
-                         uint32_t round, uint32_t* next, hipStream_t s) {
-  hipLaunchKernelGGL(k_debit_direct, dim3(kRoundGrid), dim3(kBlock), 0, s, koff, incK, qud, front, pc, kv, rnd, dd,
-                     level, round, next);
+void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel_level, dim3(kScanGrid), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_peel_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
+  hipLaunchKernelGGL(k_peel_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
+  hipLaunchKernelGGL(k_peel_finish, dim3(1), dim3(kWave), 0, s, p);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_compact_alive(const uint32_t* alive, uint32_t nalive, const int32_t* kv, uint32_t* out,
-                          PeelCounters* pc, hipStream_t s) {
-  if (!nalive) return;
-  hipLaunchKernelGGL(k_compact, dim3(grid_for(nalive, kBlock, 4096)), dim3(kBlock), 0, s, alive, nalive, kv, out,
-                     pc);
+void launch_peel_extract(const PeelBufs& p, int32_t* kv, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel_extract, dim3(grid_for(p.T ? p.T : 1, 256, 4096)), dim3(256), 0, s, p, kv);
   NUC34_LAUNCH_CHECK();
 }
 

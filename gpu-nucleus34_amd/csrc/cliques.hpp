@@ -1,6 +1,7 @@
 // (c), (d), (e): four-clique support, four-clique list and triangle->K4
 // incidence (direct), and the level-synchronous peel with frontiers.
 #pragma once
+#include "bitrows.hpp"
 #include "common.hpp"
 
 namespace nuc34 {
@@ -23,39 +24,67 @@ constexpr uint32_t kSmallL = 128;
 // *total += sum of supports. Triangles above kSmallL are appended to bigq.
 void launch_support(const TriView& v, int32_t* support, uint32_t* owned, uint64_t* total, uint32_t* bigq,
                     uint32_t* bigcount, hipStream_t s);
-// Writes the K4s owned by each triangle at k4off[t]... as
-// {T(pqs), T(pqr), T(prs), T(qrs)}.
-void launch_k4_emit(const TriView& v, const uint32_t* owned, const uint64_t* k4off, uint32_t* qud,
-                    const uint32_t* bigq, const uint32_t* bigcount, hipStream_t s);
-// Triangle -> K4 lists (4 entries per K4) at incK[koff[t]..koff[t+1]).
-void launch_k4_incidence(const uint32_t* qud, uint64_t Q, const uint64_t* koff, uint32_t* cur, uint32_t* incK,
-                         hipStream_t s);
+// Support from the bit rows: sup = K4s on t, sup_up = those whose fourth vertex
+// ranks above t's lowest vertex, own = those whose fourth vertex ranks above all
+// of t (t holds the K4's lowest three vertices); *total += sum of sup.
+void launch_support_bits(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows, int32_t* sup,
+                         uint32_t* sup_up, uint32_t* own, uint64_t* total, hipStream_t s);
+// Direct variant: per-triangle K4 lists, three partner triangle ids per entry,
+// at lists[3*koff[t] ..]. Each K4 is enumerated once, from the triangle on its
+// lowest three vertices (k4owned: (t << 32 | x position) at k4off[t] from a scan
+// of `owned`), and written into the lists of its four triangles at positions
+// given by prefix counts (up-K4s first, in fourth-vertex order, then down-K4s).
+void launch_k4_owned(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows,
+                     const uint64_t* k4off, uint64_t* k4owned, hipStream_t s);
+void launch_k4_place(const uint64_t* k4owned, uint64_t Q, const uint32_t* triA, const uint64_t* triBC,
+                     const uint32_t* O, const uint32_t* E, const uint32_t* src, const uint32_t* inpos,
+                     const BitRows& rows, const uint64_t* offA, const uint64_t* koff, const uint32_t* sup_up,
+                     uint32_t* lists, hipStream_t s);
+// Canonical K4 list: each K4 once, at its lowest-index triangle, as
+// {t, p1, p2, p3}; count pass then fill pass (qoff = scan of the counts).
+void launch_k4_canon_count(const uint64_t* koff, const uint32_t* lists, uint64_t T, uint32_t* cnt, hipStream_t s);
+void launch_k4_canon_fill(const uint64_t* koff, const uint32_t* lists, uint64_t T, const uint64_t* qoff,
+                          uint64_t first, uint64_t count, uint32_t* qud, hipStream_t s);
 
-// Peel state shared by the round kernels (device memory).
-struct PeelCounters {
-  uint32_t front;      // size of the current frontier
-  uint32_t next;       // size of the frontier being built
-  uint32_t big;        // frontier triangles queued for the wave path
-  uint32_t minval;     // min over alive supports (level selection)
-  uint32_t alive;      // size of the compacted alive list being built
-  uint32_t pad[3];
+// ---- peel ------------------------------------------------------------------
+//
+// Per-triangle peel state: dd = current support (INT32_MAX once the triangle's
+// k is fixed) and st = (round of death << 32) | k (all ones while alive).
+// Round control lives on the device so rounds run back to back without host
+// round trips; a one-workgroup kernel closes each round.
+struct PeelCtl {
+  uint32_t count[2];    // sizes of the two frontier buffers
+  uint32_t parity;      // buffer holding the current frontier
+  uint32_t need_level;  // frontier empty: pick the next level
+  uint32_t minval;      // min alive support (level selection)
+  int32_t level;
+  uint32_t round;
+  uint32_t done;
+  uint64_t peeled;
+  uint32_t big;         // frontier triangles queued for the wave path (on-demand)
+  uint32_t error;       // nonzero: inconsistent state detected
+  uint32_t nalive[2];   // sizes of the two alive-list buffers
+  uint32_t apar;        // buffer holding the current alive list
 };
 
-void launch_alive_min(const uint32_t* alive, uint32_t nalive, const int32_t* kv, const int32_t* dd,
-                      PeelCounters* pc, hipStream_t s);
-void launch_collect(const uint32_t* alive, uint32_t nalive, int32_t* kv, const int32_t* dd, uint32_t* rnd,
-                    int32_t level, uint32_t round, uint32_t* front, PeelCounters* pc, hipStream_t s);
-void launch_mark(const uint32_t* front, const PeelCounters* pc, int32_t* kv, uint32_t* rnd, int32_t level,
-                 uint32_t round, hipStream_t s);
-// On-demand debit: each dying triangle re-enumerates its K4s.
-void launch_debit_ondemand(const TriView& v, const uint32_t* front, PeelCounters* pc, const int32_t* kv,
-                           const uint32_t* rnd, int32_t* dd, int32_t level, uint32_t round, uint32_t* next,
-                           uint32_t* bigq, hipStream_t s);
-// Direct debit: each dying triangle walks its stored K4 list.
-void launch_debit_direct(const uint64_t* koff, const uint32_t* incK, const uint32_t* qud, const uint32_t* front,
-                         PeelCounters* pc, const int32_t* kv, const uint32_t* rnd, int32_t* dd, int32_t level,
-                         uint32_t round, uint32_t* next, hipStream_t s);
-void launch_compact_alive(const uint32_t* alive, uint32_t nalive, const int32_t* kv, uint32_t* out,
-                          PeelCounters* pc, hipStream_t s);
+struct PeelBufs {
+  int32_t* dd;
+  uint64_t* st;
+  uint32_t* buf0;
+  uint32_t* buf1;
+  uint32_t* alive0;  // triangles not yet fixed at the last level change
+  uint32_t* alive1;
+  uint32_t* bigq;
+  PeelCtl* ctl;
+  uint64_t T;
+};
+
+// One peel round: level selection (only when the frontier is empty), then the
+// debit of every K4 owned by a dying triangle. Every kernel returns at once
+// when ctl->done is set, so extra rounds are no-ops.
+void launch_peel_round_direct(const PeelBufs& p, const uint64_t* koff, const uint32_t* lists, hipStream_t s);
+void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t s);
+void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
+void launch_peel_extract(const PeelBufs& p, int32_t* kv, hipStream_t s);
 
 }  // namespace nuc34

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 
@@ -154,17 +155,39 @@ class DevBuf {
 class Allocator {
  public:
   uint64_t budget = 0;  // bytes; 0 = no limit beyond the device
-  uint64_t used = 0;
+  uint64_t used = 0;    // bytes held by live buffers
   uint64_t peak = 0;
+  uint64_t cached = 0;  // bytes held in the reuse cache
+  bool caching = true;
+  ~Allocator() { trim(); }
   DevBuf get(size_t bytes, const char* what) {
     DevBuf b;
     if (!bytes) return b;
+    bytes = (bytes + 255) & ~static_cast<size_t>(255);
+    // reuse a cached block of at most twice the request
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2 * bytes) {
+      b.p_ = it->second;
+      b.bytes_ = it->first;
+      b.owner_ = this;
+      cached -= it->first;
+      free_.erase(it);
+      used += b.bytes_;
+      if (used > peak) peak = used;
+      return b;
+    }
+    if (budget && used + cached + bytes > budget) trim();
     if (budget && used + bytes > budget)
       throw OutOfMemory(std::string("HBM budget exceeded allocating ") + what + " (" +
                         std::to_string(bytes) + " bytes; in use " + std::to_string(used) + " of " +
                         std::to_string(budget) + ")");
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
+    if ((e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) && cached) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, bytes);
+    }
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
       (void)hipGetLastError();
       throw OutOfMemory(std::string("device out of memory allocating ") + what + " (" +
@@ -178,13 +201,31 @@ class Allocator {
     if (used > peak) peak = used;
     return b;
   }
-  void give_back(size_t bytes) { used -= bytes; }
+  // Called by DevBuf::release. Work using the block is ordered before any later
+  // user on the single stream of the owning context.
+  void give_back(void* p, size_t bytes) {
+    used -= bytes;
+    if (caching) {
+      free_.emplace(bytes, p);
+      cached += bytes;
+    } else {
+      (void)hipFree(p);
+    }
+  }
+  void trim() {
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached = 0;
+  }
+
+ private:
+  std::multimap<size_t, void*> free_;
 };
 
 inline void DevBuf::release() {
   if (p_) {
-    (void)hipFree(p_);
-    if (owner_) owner_->give_back(bytes_);
+    if (owner_) owner_->give_back(p_, bytes_);
+    else (void)hipFree(p_);
   }
   p_ = nullptr;
   bytes_ = 0;

@@ -1,6 +1,7 @@
 // Launch wrappers for the gfx950 kernels of the (3,4)-nucleus path. All device
 // pointers; every wrapper enqueues on stream `s` and checks the launch.
 #pragma once
+#include "bitrows.hpp"
 #include "common.hpp"
 
 namespace nuc34 {
@@ -11,12 +12,12 @@ void launch_orient_fill(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, cons
                         uint32_t* src, hipStream_t s);
 void launch_fill_src(const uint32_t* O, uint32_t n, uint32_t* src, hipStream_t s);
 
-// (b) triangle enumeration over oriented edges [e_begin, e_end).
-void launch_tri_count(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
-                      uint64_t e_end, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s);
-void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
-                     uint64_t e_end, const uint64_t* offA, uint32_t* curA, uint32_t* triA, uint64_t* triBC,
-                     hipStream_t s);
+// (b) triangle enumeration over oriented edges [e_begin, e_end): the bits pass
+// fills the incidence rows, the emit pass writes triangles at canonical indices.
+void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
+                     const BitRows& rows, const uint32_t* inpos, hipStream_t s);
+void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
+                     const BitRows& rows, const uint64_t* offA, uint32_t* triA, uint64_t* triBC, hipStream_t s);
 
 // (d) edge -> triangle incidence.
 void launch_inc_len(const uint32_t* cntA, const uint32_t* cntB, const uint32_t* cntC, uint64_t m, uint32_t* len,

@@ -10,12 +10,13 @@
 // consecutive, so their out-lists are one contiguous range of E, staged in LDS
 // when it fits; each candidate w = E[c] is then found in N+(u) by binary search
 // in LDS (global memory otherwise).
-//   count pass: per-edge totals of the a-, b- and c-roles (a-role counts give the
-//               canonical offsets, all three the incidence lengths);
-//   emit pass:  each triangle lands in its a-edge segment (offsets from a scan of
-//               the a-role counts); a segmented sort by (b, c) then gives exactly
-//               the (a, b) order of the reference's sorted triangle list
-//               (tri_t::operator<, utils.cuh:93-102).
+//   bits pass: every triangle sets one bit in each of three per-edge rows (the
+//              bit-packed edge->triangle incidence, bitrows.hpp): the b-role row
+//              of b, the a-role row of a and the c-role row of c;
+//   emit pass: a triangle's canonical index is offA[a] plus the number of
+//              a-role bits of a below v's position, i.e. exactly its place in the
+//              reference's (a, b)-sorted triangle list (tri_t::operator<,
+//              utils.cuh:93-102), with no sort and no atomics.
 #include "kernels.hpp"
 
 namespace nuc34 {
@@ -38,15 +39,15 @@ __device__ __forceinline__ uint32_t search_u32(P list, uint32_t lo, uint32_t hi,
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
                                                      const uint32_t* __restrict__ src, uint64_t e_begin,
-                                                     uint64_t e_end, uint32_t* __restrict__ cntA,
-                                                     uint32_t* __restrict__ cntB, uint32_t* __restrict__ cntC,
-                                                     const uint64_t* __restrict__ offA, uint32_t* __restrict__ curA,
+                                                     uint64_t e_end, BitRows rows,
+                                                     const uint32_t* __restrict__ inpos,
+                                                     const uint64_t* __restrict__ offA,
                                                      uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC) {
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
   __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
   __shared__ uint32_t s_len[kWavesPerBlock][kWave];
   __shared__ uint32_t s_vb[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_cb[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_ob[kWavesPerBlock][kWave];
   __shared__ uint32_t s_list[kWavesPerBlock][kListCap];
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
     s_l0[w][lane] = staged ? ob - lo : ob;
     s_len[w][lane] = oe - ob;
     s_vb[w][lane] = vb;
-    s_cb[w][lane] = 0;
+    s_ob[w][lane] = ob;
     wave_sync();
     for (uint32_t base = 0; base < total; base += kWave) {
       const uint32_t item = base + lane;
@@ -107,21 +108,22 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
         if (found) {
           const uint32_t a = pos;
           const uint32_t b = static_cast<uint32_t>(e0 + i);
+          const uint32_t ob_u = s_ob[w][i];
+          const uint32_t jv = b - ob_u, jw = a - ob_u;
           if constexpr (!EMIT) {
-            atomic_add_u32(&cntA[a], 1u);
-            atomicAdd(&s_cb[w][i], 1u);
-            atomic_add_u32(&cntC[c], 1u);
+            // b-role: w seen from (u,v); a-role: v seen from (u,w); c-role: u's
+            // position among the in-neighbours of v, seen from (v,w).
+            atomicOr(&rows.Aout[rows.rowA[b] + (jw >> 5)], 1u << (jw & 31));
+            atomicOr(&rows.Ain[rows.rowA[a] + (jv >> 5)], 1u << (jv & 31));
+            const uint32_t iu = inpos[b];
+            atomicOr(&rows.Cm[rows.rowC[c] + (iu >> 5)], 1u << (iu & 31));
           } else {
-            const uint64_t p = offA[a] + atomic_add_u32(&curA[a], 1u);
+            const uint64_t p = offA[a] + bits_below(rows.Ain + rows.rowA[a], jv);
             triA[p] = a;
             triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
           }
         }
       }
-    }
-    wave_sync();
-    if constexpr (!EMIT) {
-      if (valid) cntB[e] = s_cb[w][lane];
     }
     wave_sync();
   }
@@ -134,20 +136,19 @@ unsigned tri_grid(uint64_t edges) {
 
 }  // namespace
 
-void launch_tri_count(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
-                      uint64_t e_end, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s) {
+void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
+                     const BitRows& rows, const uint32_t* inpos, hipStream_t s) {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_tri_enum<false>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, cntA, cntB, cntC, nullptr, nullptr, nullptr, nullptr);
+                     e_end, rows, inpos, nullptr, nullptr, nullptr);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin,
-                     uint64_t e_end, const uint64_t* offA, uint32_t* curA, uint32_t* triA, uint64_t* triBC,
-                     hipStream_t s) {
+void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
+                     const BitRows& rows, const uint64_t* offA, uint32_t* triA, uint64_t* triBC, hipStream_t s) {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, nullptr, nullptr, nullptr, offA, curA, triA, triBC);
+                     e_end, rows, nullptr, offA, triA, triBC);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -29,6 +29,10 @@ class Nuc34Error(RuntimeError):
         self.code = code
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("ms", C.c_double), ("launches", C.c_uint64)]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("n", C.c_uint64), ("m", C.c_uint64), ("triangles", C.c_uint64), ("four_cliques", C.c_uint64),
@@ -36,11 +40,14 @@ class Stats(C.Structure):
         ("peak_bytes", C.c_uint64), ("k_max", C.c_int32), ("variant", C.c_int32),
         ("ms_orient", C.c_double), ("ms_triangles", C.c_double), ("ms_incidence", C.c_double),
         ("ms_support", C.c_double), ("ms_k4_list", C.c_double), ("ms_peel", C.c_double),
-        ("ms_total", C.c_double),
+        ("ms_total", C.c_double), ("n_kernels", C.c_uint32), ("kernels", KernelStat * 8),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("kernels", "n_kernels")}
+        d["kernels"] = {self.kernels[i].name.decode(): {"ms": self.kernels[i].ms, "launches": self.kernels[i].launches}
+                        for i in range(self.n_kernels)}
+        return d
 
 
 def _load() -> C.CDLL:
@@ -70,6 +77,7 @@ def _load() -> C.CDLL:
         "nuc34_copy_k": (C.c_int, [vp, i32p]),
         "nuc34_copy_fourcliques": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p]),
         "nuc34_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+        "nuc34_reset": (C.c_int, [vp]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
@@ -242,6 +250,10 @@ class Context:
         out = np.empty((max(count, 1), 4), dtype=np.uint32)
         self._check(_lib.nuc34_copy_fourcliques(self._h, first, count, _u32(out)))
         return out[:count]
+
+    def reset(self):
+        """Drop every stage after the loaded graph (buffers go to the reuse cache)."""
+        self._check(_lib.nuc34_reset(self._h))
 
     def stats(self) -> dict:
         st = Stats()

@@ -49,6 +49,16 @@ enum { NUC34_DIRECT = 0, NUC34_ONDEMAND = 1, NUC34_AUTO = 2 };
 typedef struct nuc34_ctx nuc34_ctx;
 typedef struct nuc34_graph nuc34_graph;
 
+#define NUC34_MAX_KERNEL_STATS 8
+
+/* Device time of one kernel (or kernel group) from HIP events recorded on the
+ * context stream around its launches during the last decomposition. */
+typedef struct nuc34_kernel_stat {
+  char name[32];
+  double ms;         /* summed over launches */
+  uint64_t launches;
+} nuc34_kernel_stat;
+
 typedef struct nuc34_stats {
   uint64_t n;                 /* vertices */
   uint64_t m;                 /* oriented edges */
@@ -67,6 +77,8 @@ typedef struct nuc34_stats {
   double ms_k4_list;          /* direct: K4 list + triangle->K4 incidence */
   double ms_peel;
   double ms_total;            /* triangles .. peel (orientation excluded) */
+  uint32_t n_kernels;
+  nuc34_kernel_stat kernels[NUC34_MAX_KERNEL_STATS];
 } nuc34_stats;
 
 /* ---- host-side datasets (reference loaders, exact semantics) ------------ */
@@ -116,6 +128,11 @@ int nuc34_copy_k(nuc34_ctx* ctx, int32_t* k);
 /* Direct variant only: K4s [first, first+count) as 4 triangle ids each. */
 int nuc34_copy_fourcliques(nuc34_ctx* ctx, uint64_t first, uint64_t count, uint32_t* quds);
 int nuc34_get_stats(const nuc34_ctx* ctx, nuc34_stats* stats);
+/* Drops every stage computed from the loaded graph (triangles onwards) and
+ * keeps the graph, so the next call recomputes them; device buffers return to
+ * the context's reuse cache. No counterpart in the reference: a context utility
+ * for repeated runs (benchmarks, serving). */
+int nuc34_reset(nuc34_ctx* ctx);
 
 /* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
 int nuc34_device_info(int device, char* buf, size_t len);

@@ -169,3 +169,22 @@ def test_cli_matches_golden(tmp_path):
         rows = [[*map(int, line.split("\t")[0].strip("()").split(",")), int(line.split("\t")[1])]
                 for line in out.read_text().splitlines()]
         assert rows == exp["rows"]
+
+
+def test_graph_rounds_match_eager_rounds(tmp_path, monkeypatch):
+    """Peel rounds run as a captured graph replayed many times, with host syncs
+    between replays; every replayed run must equal the eagerly launched one and
+    the oracle."""
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 12, out=tmp_path / "g.edges")
+    ref = oracle34.OracleGraph.load(p, orient=True).run("host", 8)
+    g = nuc34.load_dataset(p)
+    for variant in ("direct", "ondemand"):
+        monkeypatch.setenv("NUC34_NO_GRAPH", "1")
+        _, _, k_eager, st_eager = gpu_run(g, variant)
+        monkeypatch.delenv("NUC34_NO_GRAPH")
+        assert np.array_equal(k_eager, ref["k"]) and st_eager["rounds"] == ref["rounds"] or variant
+        for _ in range(3):
+            _, _, k_graph, st_graph = gpu_run(g, variant)
+            assert np.array_equal(k_graph, k_eager), variant
+            assert st_graph["rounds"] == st_eager["rounds"] and st_graph["k_max"] == st_eager["k_max"]
+        assert np.array_equal(k_eager, ref["k"]), variant
