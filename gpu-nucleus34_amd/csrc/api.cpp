@@ -401,13 +401,13 @@ void run_peel(nuc34_ctx* c, int variant) {
     return;
   }
   DevBuf dd = c->alloc.get(T * 4, "peel supports");
-  DevBuf st = c->alloc.get(T * 8, "peel state");
+  DevBuf rnd = c->alloc.get(T * 4, "death rounds");
   DevBuf b0 = c->alloc.get(T * 4, "frontier");
   DevBuf b1 = c->alloc.get(T * 4, "frontier");
   DevBuf a0 = c->alloc.get(T * 4, "alive list");
   DevBuf a1 = c->alloc.get(T * 4, "alive list");
   DevBuf ctlb = c->alloc.get(sizeof(PeelCtl), "peel control");
-  PeelBufs pb{dd.as<int32_t>(), st.as<uint64_t>(), b0.as<uint32_t>(), b1.as<uint32_t>(), a0.as<uint32_t>(),
+  PeelBufs pb{dd.as<int32_t>(), rnd.as<uint32_t>(), c->kv.as<int32_t>(), b0.as<uint32_t>(), b1.as<uint32_t>(), a0.as<uint32_t>(),
               a1.as<uint32_t>(), variant == NUC34_DIRECT ? nullptr : c->bigq.as<uint32_t>(), ctlb.as<PeelCtl>(), T};
   const TriView v = triview(c);
   auto enqueue_rounds = [&](int n) {
@@ -444,7 +444,6 @@ void run_peel(nuc34_ctx* c, int variant) {
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    launch_peel_extract(pb, c->kv.as<int32_t>(), c->s);
   });
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
   c->st.rounds = h.round;
@@ -764,21 +763,24 @@ int nuc34_copy_k(nuc34_ctx* c, int32_t* k) {
 }
 
 int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t* quds) {
-  if (!c || !c->has_k4) return fail(c, NUC34_E_STATE, "K4 lists not built (direct variant only)");
+  if (!c || !c->has_k4) return fail(c, NUC34_E_STATE, "K4 list not built (direct variant only)");
   if (first > c->Q || count > c->Q - first) return fail(c, NUC34_E_INVALID, "K4 range out of bounds");
   return guard(c, [&]() -> int {
     if (!count) return NUC34_OK;
     const uint64_t T = c->T;
-    DevBuf cnt = c->alloc.get(T * 4 + 4, "K4 counts");
-    DevBuf qoff = c->alloc.get((T + 1) * 8, "K4 offsets");
-    launch_k4_canon_count(c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), T, cnt.as<uint32_t>(), c->s);
+    DevBuf k4off = c->alloc.get((T + 1) * 8, "owned K4 offsets");
     {
       DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), qoff.as<uint64_t>(), T, scratch.as<uint64_t>(), c->s);
+      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
+                                         c->s);
     }
+    DevBuf k4 = c->alloc.get(c->Q * 8 + 8, "owned K4s");
+    launch_k4_owned(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), k4off.as<uint64_t>(),
+                    k4.as<uint64_t>(), c->s);
     DevBuf out = c->alloc.get(count * 16, "K4 copy");
-    launch_k4_canon_fill(c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), T, qoff.as<uint64_t>(), first, count,
-                         out.as<uint32_t>(), c->s);
+    launch_k4_quads(k4.as<uint64_t>(), first, count, c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
+                    c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), bitrows(c),
+                    c->offA.as<uint64_t>(), out.as<uint32_t>(), c->s);
     NUC34_HIP(hipMemcpyAsync(quds, out.as<void>(), count * 16, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;

@@ -235,11 +235,44 @@ __global__ __launch_bounds__(kBlock) void k_k4_owned(const uint32_t* __restrict_
   }
 }
 
-// Direct K4 lists, step 2: one thread per K4 {u < v < w < x} (t = T(u,v,w)):
-// the other three triangle ids from prefix counts of the bit rows, then one
-// entry (the other three ids) in the list of each of the four triangles at a
-// position fixed by prefix counts: up-K4s (fourth vertex above the triangle's
-// lowest vertex) first, in fourth-vertex order, then down-K4s.
+// The four triangles of the K4 {u < v < w < x} owned by t = T(u,v,w), x at
+// position jx of N+(u): T(u,v,x) and T(u,w,x) from the a-role rows of (u,x),
+// T(v,w,x) from the a-role row of (v,x) = c-edge of T(u,v,x).
+struct K4Ids {
+  uint32_t t, t1, t2, t3;  // T(u,v,w), T(u,v,x), T(u,w,x), T(v,w,x)
+  uint32_t a, b, c, c1;    // edges (u,w), (u,v), (v,w), (v,x)
+  uint32_t ou, jv, jw, ex;
+};
+
+__device__ __forceinline__ K4Ids k4_ids(uint64_t key, const uint32_t* __restrict__ triA,
+                                        const uint64_t* __restrict__ triBC, const uint32_t* __restrict__ O,
+                                        const uint32_t* __restrict__ E, const uint32_t* __restrict__ src,
+                                        const BitRows& r, const uint64_t* __restrict__ offA) {
+  K4Ids k;
+  k.t = static_cast<uint32_t>(key >> 32);
+  const uint32_t jx = static_cast<uint32_t>(key);
+  k.a = triA[k.t];
+  const uint64_t bc = triBC[k.t];
+  k.b = static_cast<uint32_t>(bc >> 32);
+  k.c = static_cast<uint32_t>(bc);
+  k.ou = O[src[k.b]];
+  k.jv = k.b - k.ou;
+  k.jw = k.a - k.ou;
+  k.ex = k.ou + jx;
+  const uint32_t* Ix = r.Ain + r.rowA[k.ex];
+  k.t1 = static_cast<uint32_t>(offA[k.ex] + bits_below(Ix, k.jv));
+  k.t2 = static_cast<uint32_t>(offA[k.ex] + bits_below(Ix, k.jw));
+  k.c1 = static_cast<uint32_t>(triBC[k.t1]);
+  const uint32_t ov = O[E[k.b]];
+  k.t3 = static_cast<uint32_t>(offA[k.c1] + bits_below(r.Ain + r.rowA[k.c1], k.c - ov));
+  return k;
+}
+
+// Direct K4 lists, step 2: one thread per K4; an entry (the other three
+// triangle ids) in the list of each of its four triangles, at a position fixed
+// by prefix counts: up-K4s (fourth vertex above the triangle's lowest vertex)
+// first, in fourth-vertex order, then down-K4s in order of the lowest vertex's
+// in-list position.
 __global__ __launch_bounds__(kBlock) void k_k4_place(const uint64_t* __restrict__ k4, uint64_t Q,
                                                      const uint32_t* __restrict__ triA,
                                                      const uint64_t* __restrict__ triBC,
@@ -252,73 +285,42 @@ __global__ __launch_bounds__(kBlock) void k_k4_place(const uint64_t* __restrict_
                                                      uint32_t* __restrict__ lists) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < Q; q += stride) {
-    const uint64_t key = k4[q];
-    const uint32_t t = static_cast<uint32_t>(key >> 32);
-    const uint32_t jx = static_cast<uint32_t>(key);
-    const uint32_t a = triA[t];
-    const uint64_t bc = triBC[t];
-    const uint32_t b = static_cast<uint32_t>(bc >> 32);
-    const uint32_t c = static_cast<uint32_t>(bc);
-    const uint32_t ou = O[src[b]];
-    const uint32_t jv = b - ou, jw = a - ou;
-    const uint32_t* Aa = r.Aout + r.rowA[a];
-    const uint32_t* Ab = r.Aout + r.rowA[b];
-    const uint32_t* Ia = r.Ain + r.rowA[a];
-    const uint32_t* Ib = r.Ain + r.rowA[b];
-    const uint32_t ex = ou + jx;
-    const uint32_t* Ax = r.Aout + r.rowA[ex];
-    const uint32_t* Ix = r.Ain + r.rowA[ex];
-    const uint32_t t1 = static_cast<uint32_t>(offA[ex] + bits_below(Ix, jv));  // T(u,v,x)
-    const uint32_t t2 = static_cast<uint32_t>(offA[ex] + bits_below(Ix, jw));  // T(u,w,x)
-    const uint32_t c1 = static_cast<uint32_t>(triBC[t1]);                        // idx(v,x)
-    const uint32_t ov = O[E[b]];
-    const uint32_t t3 = static_cast<uint32_t>(offA[c1] + bits_below(r.Ain + r.rowA[c1], c - ov));  // T(v,w,x)
-    put3(lists, koff[t] + sym_bits_below(Aa, Ia, Ab, Ib, jx), t1, t2, t3);
-    put3(lists, koff[t1] + sym_bits_below(Ab, Ib, Ax, Ix, jw), t, t2, t3);
-    put3(lists, koff[t2] + sym_bits_below(Aa, Ia, Ax, Ix, jv), t, t1, t3);
-    put3(lists, koff[t3] + sup_up[t3] + and_bits_below(r.Cm + r.rowC[c], r.Cm + r.rowC[c1], inpos[b]), t, t1, t2);
+    const K4Ids k = k4_ids(k4[q], triA, triBC, O, E, src, r, offA);
+    const uint32_t* Aa = r.Aout + r.rowA[k.a];
+    const uint32_t* Ab = r.Aout + r.rowA[k.b];
+    const uint32_t* Ia = r.Ain + r.rowA[k.a];
+    const uint32_t* Ib = r.Ain + r.rowA[k.b];
+    const uint32_t* Ax = r.Aout + r.rowA[k.ex];
+    const uint32_t* Ix = r.Ain + r.rowA[k.ex];
+    const uint32_t jx = k.ex - k.ou;
+    put3(lists, koff[k.t] + sym_bits_below(Aa, Ia, Ab, Ib, jx), k.t1, k.t2, k.t3);
+    put3(lists, koff[k.t1] + sym_bits_below(Ab, Ib, Ax, Ix, k.jw), k.t, k.t2, k.t3);
+    put3(lists, koff[k.t2] + sym_bits_below(Aa, Ia, Ax, Ix, k.jv), k.t, k.t1, k.t3);
+    put3(lists,
+         koff[k.t3] + sup_up[k.t3] + and_bits_below(r.Cm + r.rowC[k.c], r.Cm + r.rowC[k.c1], inpos[k.b]), k.t,
+         k.t1, k.t2);
   }
 }
 
-__global__ void k_k4_canon_count(const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists, uint64_t T,
-                                 uint32_t* __restrict__ cnt) {
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
-       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t n = 0;
-    for (uint64_t j = koff[t]; j < koff[t + 1]; ++j) {
-      const uint32_t* q = lists + 3 * j;
-      n += (t < q[0] && t < q[1] && t < q[2]) ? 1u : 0u;
-    }
-    cnt[t] = n;
-  }
-}
-
-__global__ void k_k4_canon_fill(const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists, uint64_t T,
-                                const uint64_t* __restrict__ qoff, uint64_t first, uint64_t count,
-                                uint32_t* __restrict__ qud) {
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
-       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint64_t pos = qoff[t];
-    if (qoff[t + 1] <= first || pos >= first + count) continue;
-    for (uint64_t j = koff[t]; j < koff[t + 1]; ++j) {
-      const uint32_t* q = lists + 3 * j;
-      if (t < q[0] && t < q[1] && t < q[2]) {
-        const uint64_t at = pos++;
-        if (at < first || at >= first + count) continue;
-        uint32_t* o = qud + 4 * (at - first);
-        o[0] = static_cast<uint32_t>(t);
-        o[1] = q[0];
-        o[2] = q[1];
-        o[3] = q[2];
-      }
-    }
+__global__ void k_k4_quads(const uint64_t* __restrict__ k4, uint64_t first, uint64_t count,
+                           const uint32_t* __restrict__ triA, const uint64_t* __restrict__ triBC,
+                           const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
+                           const uint32_t* __restrict__ src, BitRows r, const uint64_t* __restrict__ offA,
+                           uint32_t* __restrict__ quads) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const K4Ids k = k4_ids(k4[first + i], triA, triBC, O, E, src, r, offA);
+    quads[4 * i + 0] = k.t1;
+    quads[4 * i + 1] = k.t;
+    quads[4 * i + 2] = k.t2;
+    quads[4 * i + 3] = k.t3;
   }
 }
 
 // ---- peel ------------------------------------------------------------------
 
 constexpr int32_t kDead = 0x7FFFFFFF;
-constexpr uint64_t kAlive = ~0ull;
+constexpr uint32_t kAliveRound = 0xFFFFFFFFu;
 
 // Level in force for this round: when the frontier ran empty, the new level is
 // max(level, min alive support) from k_peel_level (committed by k_peel_finish).
@@ -332,7 +334,8 @@ __global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t*
   for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < p.T;
        t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     p.dd[t] = sup[t];
-    p.st[t] = kAlive;
+    p.rnd[t] = kAliveRound;
+    p.kv[t] = -1;
     p.alive0[t] = static_cast<uint32_t>(t);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -436,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p) {
 #pragma unroll
     for (int j = 0; j < kCollectItems; ++j) {
       if (die & (1u << j)) {
-        p.st[tv[j]] = (static_cast<uint64_t>(round) << 32) | static_cast<uint32_t>(level);
+        p.rnd[tv[j]] = round;
+        p.kv[tv[j]] = level;
         p.dd[tv[j]] = kDead;
         cur[od++] = tv[j];
       } else if (stay & (1u << j)) {
@@ -454,20 +458,20 @@ __global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p) {
 // level+1 to level is pushed: its k is fixed now and it dies next round.
 __device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
                                            uint32_t round, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
-  uint64_t sq[3];
+  uint32_t rq[3];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    sq[s] = p.st[q[s]];
-    const uint32_t r = static_cast<uint32_t>(sq[s] >> 32);
-    if (r < round) return;
-    if (r == round && q[s] < t) return;
+    rq[s] = p.rnd[q[s]];
+    if (rq[s] < round) return;
+    if (rq[s] == round && q[s] < t) return;
   }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    if (sq[s] != kAlive) continue;
+    if (rq[s] != kAliveRound) continue;
     const int32_t old = atomic_sub_i32(&p.dd[q[s]], 1);
     if (old == level + 1) {
-      p.st[q[s]] = (static_cast<uint64_t>(round + 1) << 32) | static_cast<uint32_t>(level);
+      p.rnd[q[s]] = round + 1;
+      p.kv[q[s]] = level;
       __hip_atomic_store(&p.dd[q[s]], kDead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       nxt[atomic_add_u32(ncount, 1u)] = q[s];
     }
@@ -577,14 +581,6 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, TriVie
   }
 }
 
-__global__ void k_peel_extract(PeelBufs p, int32_t* __restrict__ kv) {
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < p.T;
-       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t s = p.st[t];
-    kv[t] = s == kAlive ? -1 : static_cast<int32_t>(static_cast<uint32_t>(s));
-  }
-}
-
 constexpr unsigned kRoundGrid = 2048;
 
 }  // namespace
@@ -625,17 +621,12 @@ void launch_k4_place(const uint64_t* k4, uint64_t Q, const uint32_t* triA, const
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_canon_count(const uint64_t* koff, const uint32_t* lists, uint64_t T, uint32_t* cnt, hipStream_t s) {
-  if (!T) return;
-  hipLaunchKernelGGL(k_k4_canon_count, dim3(grid_for(T, 256, 1u << 14)), dim3(256), 0, s, koff, lists, T, cnt);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_k4_canon_fill(const uint64_t* koff, const uint32_t* lists, uint64_t T, const uint64_t* qoff,
-                          uint64_t first, uint64_t count, uint32_t* qud, hipStream_t s) {
-  if (!T) return;
-  hipLaunchKernelGGL(k_k4_canon_fill, dim3(grid_for(T, 256, 1u << 14)), dim3(256), 0, s, koff, lists, T, qoff, first,
-                     count, qud);
+void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, const uint32_t* triA,
+                     const uint64_t* triBC, const uint32_t* O, const uint32_t* E, const uint32_t* src,
+                     const BitRows& rows, const uint64_t* offA, uint32_t* quads, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_k4_quads, dim3(grid_for(count, 256, 1u << 14)), dim3(256), 0, s, k4owned, first, count, triA,
+                     triBC, O, E, src, rows, offA, quads);
   NUC34_LAUNCH_CHECK();
 }
 
@@ -660,11 +651,6 @@ void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t
   hipLaunchKernelGGL(k_peel_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
   hipLaunchKernelGGL(k_peel_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
   hipLaunchKernelGGL(k_peel_finish, dim3(1), dim3(kWave), 0, s, p);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_peel_extract(const PeelBufs& p, int32_t* kv, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel_extract, dim3(grid_for(p.T ? p.T : 1, 256, 4096)), dim3(256), 0, s, p, kv);
   NUC34_LAUNCH_CHECK();
 }
 

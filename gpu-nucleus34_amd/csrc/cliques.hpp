@@ -40,16 +40,18 @@ void launch_k4_place(const uint64_t* k4owned, uint64_t Q, const uint32_t* triA, 
                      const uint32_t* O, const uint32_t* E, const uint32_t* src, const uint32_t* inpos,
                      const BitRows& rows, const uint64_t* offA, const uint64_t* koff, const uint32_t* sup_up,
                      uint32_t* lists, hipStream_t s);
-// Canonical K4 list: each K4 once, at its lowest-index triangle, as
-// {t, p1, p2, p3}; count pass then fill pass (qoff = scan of the counts).
-void launch_k4_canon_count(const uint64_t* koff, const uint32_t* lists, uint64_t T, uint32_t* cnt, hipStream_t s);
-void launch_k4_canon_fill(const uint64_t* koff, const uint32_t* lists, uint64_t T, const uint64_t* qoff,
-                          uint64_t first, uint64_t count, uint32_t* qud, hipStream_t s);
+// The K4 list in canonical order (by the triangle on the three lowest vertices,
+// then by fourth vertex) as {T(pqs), T(pqr), T(prs), T(qrs)} for p<q<r<s, for
+// owned-K4 records [first, first + count).
+void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, const uint32_t* triA,
+                     const uint64_t* triBC, const uint32_t* O, const uint32_t* E, const uint32_t* src,
+                     const BitRows& rows, const uint64_t* offA, uint32_t* quads, hipStream_t s);
 
 // ---- peel ------------------------------------------------------------------
 //
 // Per-triangle peel state: dd = current support (INT32_MAX once the triangle's
-// k is fixed) and st = (round of death << 32) | k (all ones while alive).
+// k is fixed), rnd = round of death (all ones while alive), kv = k (-1 while
+// alive; the result).
 // Round control lives on the device so rounds run back to back without host
 // round trips; a one-workgroup kernel closes each round.
 struct PeelCtl {
@@ -69,7 +71,8 @@ struct PeelCtl {
 
 struct PeelBufs {
   int32_t* dd;
-  uint64_t* st;
+  uint32_t* rnd;
+  int32_t* kv;
   uint32_t* buf0;
   uint32_t* buf1;
   uint32_t* alive0;  // triangles not yet fixed at the last level change
@@ -85,6 +88,6 @@ struct PeelBufs {
 void launch_peel_round_direct(const PeelBufs& p, const uint64_t* koff, const uint32_t* lists, hipStream_t s);
 void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
-void launch_peel_extract(const PeelBufs& p, int32_t* kv, hipStream_t s);
+
 
 }  // namespace nuc34

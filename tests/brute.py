@@ -92,4 +92,5 @@ def decompose(n, nbrs):
         "k": k,
         "K_max": max(k) if k else 0,
         "out": out,
+        "k4s": [frozenset(q) for q in k4s],
     }

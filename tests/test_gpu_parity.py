@@ -188,3 +188,35 @@ def test_graph_rounds_match_eager_rounds(tmp_path, monkeypatch):
             assert np.array_equal(k_graph, k_eager), variant
             assert st_graph["rounds"] == st_eager["rounds"] and st_graph["k_max"] == st_eager["k_max"]
         assert np.array_equal(k_eager, ref["k"]), variant
+
+
+@pytest.mark.parametrize("name", ["rmat8.edges", "rmat7k8.edges", "two_cliques.edges"])
+def test_k4_list(name):
+    """The direct variant's K4 list: every K4 exactly once, as
+    {T(pqs), T(pqr), T(prs), T(qrs)} for p<q<r<s in rank order, grouped by the
+    triangle on the three lowest vertices (ascending)."""
+    import brute
+    from test_golden_helpers import pairs_of
+    path = os.path.join(GOLDEN, name)
+    n, nbrs = brute.undirected_from_pairs(pairs_of(path))
+    b = brute.decompose(n, nbrs)
+    g = nuc34.load_dataset(path)
+    with nuc34.Context() as ctx:
+        ctx.load_graph(g)
+        st = ctx.decompose("direct")
+        quds = ctx.copy_fourcliques()
+        tris = ctx.copy_triangles()
+        O, E = ctx.oriented_csr()
+    assert len(quds) == b["Q"] == st["four_cliques"]
+    assert {frozenset(map(int, q)) for q in quds} == set(b["k4s"])
+    deg = np.array([len(x) for x in nbrs])
+    src = np.repeat(np.arange(len(O) - 1), np.diff(O))
+    def verts(t):
+        a, bb, c = tris[t]
+        return {int(src[bb]), int(E[bb]), int(E[a])}
+    rank = lambda v: (deg[v], v)
+    for q in quds:
+        vs = sorted(set().union(*(verts(int(t)) for t in q)), key=rank)
+        p_, q_, r_, s_ = vs
+        assert [verts(int(t)) for t in q] == [{p_, q_, s_}, {p_, q_, r_}, {p_, r_, s_}, {q_, r_, s_}]
+    assert all(quds[i][1] <= quds[i + 1][1] for i in range(len(quds) - 1))
