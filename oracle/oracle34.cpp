@@ -513,31 +513,49 @@ Result ondemand(const Graph& g, size_t parts) {
   // Level-synchronous peel (cuNucleus34_ondemand.cu:599-690): level = max(level,
   // min alive support); dying = alive with support <= level; a K4 belongs to its
   // first member to die, same-round ties to the lowest index; owners debit the
-  // surviving members.
+  // surviving members. Rounds after the first of a level only need the
+  // triangles whose support fell from level+1 to level in the previous round:
+  // every other alive triangle still has support > level, so this is the same
+  // dying set the reference's full rescan finds, round for round.
   std::vector<int32_t> kv(T, -1);
   std::vector<uint32_t> round_of(T, 0xFFFFFFFFu);
-  std::vector<uint32_t> alive(T), dying;
+  std::vector<uint32_t> alive(T), dying, crossed;
   std::iota(alive.begin(), alive.end(), 0u);
   int32_t level = 0;
   size_t peeled = 0;
   uint32_t round = 0;
-  size_t last_compact = T;
+  const int nth = omp_get_max_threads();
+  std::vector<std::vector<uint32_t>> local(static_cast<size_t>(nth));
   while (peeled < T) {
-    int32_t mn = std::numeric_limits<int32_t>::max();
-    for (uint32_t t : alive)
-      if (kv[t] == -1) mn = std::min(mn, std::max(DD[t], 0));
-    level = std::max(level, mn);
-    dying.clear();
-    for (uint32_t t : alive) {
-      if (kv[t] == -1 && DD[t] <= level) {
-        kv[t] = level;
-        round_of[t] = round;
-        dying.push_back(t);
+    if (crossed.empty()) {
+      int32_t mn = std::numeric_limits<int32_t>::max();
+#pragma omp parallel for reduction(min : mn)
+      for (size_t i = 0; i < alive.size(); ++i) {
+        const uint32_t t = alive[i];
+        if (kv[t] == -1) mn = std::min(mn, std::max(DD[t], 0));
       }
+      level = std::max(level, mn);
+      dying.clear();
+      std::vector<uint32_t> keep;
+      for (uint32_t t : alive) {
+        if (kv[t] != -1) continue;
+        if (DD[t] <= level) dying.push_back(t);
+        else keep.push_back(t);
+      }
+      alive.swap(keep);
+    } else {
+      dying.swap(crossed);
+      crossed.clear();
     }
+    for (uint32_t t : dying) {
+      kv[t] = level;
+      round_of[t] = round;
+    }
+    for (auto& l : local) l.clear();
 #pragma omp parallel for schedule(dynamic, 64)
     for (size_t di = 0; di < dying.size(); ++di) {
       const uint32_t t = dying[di];
+      auto& out = local[static_cast<size_t>(omp_get_thread_num())];
       for_each_clique(t, [&](uint32_t p1, uint32_t p2, uint32_t p3) {
         const uint32_t p[3] = {p1, p2, p3};
         for (uint32_t x : p) {
@@ -546,18 +564,12 @@ Result ondemand(const Graph& g, size_t parts) {
           if (x < t) return;
         }
         for (uint32_t x : p)
-          if (kv[x] == -1) __atomic_fetch_sub(&DD[x], 1, __ATOMIC_RELAXED);
+          if (kv[x] == -1 && __atomic_fetch_sub(&DD[x], 1, __ATOMIC_RELAXED) == level + 1) out.push_back(x);
       });
     }
+    for (auto& l : local) crossed.insert(crossed.end(), l.begin(), l.end());
     peeled += dying.size();
     ++round;
-    if (peeled < T && (T - peeled) * 2 <= last_compact) {
-      std::vector<uint32_t> nxt;
-      for (uint32_t t : alive)
-        if (kv[t] == -1) nxt.push_back(t);
-      alive.swap(nxt);
-      last_compact = alive.size();
-    }
   }
   auto t4 = clk::now();
   r.t_peel = secs(t3, t4);

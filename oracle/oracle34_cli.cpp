@@ -1,7 +1,9 @@
 // ORACLE — test infrastructure only. Command-line front end of the restatement:
-//   oracle34 <dataset> [--mode host|ondemand] [--parts N] [--kfile PATH]
+//   oracle34 <dataset> [--mode host|ondemand] [--parts N] [--kfile PATH] [--bin PREFIX]
 // Prints one JSON line (counts, K_max, phase seconds, threads) and optionally
-// writes the k file: "(a,b,c)\tk" per triangle in (a,b) order.
+// writes the k file: "(a,b,c)\tk" per triangle in (a,b) order. --bin writes
+// PREFIX.k (int32 k) and PREFIX.tri (uint32 a,b,c) per triangle, raw
+// little-endian in (a,b) order, for hashing results too large for text.
 #include <omp.h>
 
 #include <chrono>
@@ -17,13 +19,14 @@ int main(int argc, char** argv) {
     std::cerr << "usage: oracle34 <dataset> [--mode host|ondemand] [--parts N] [--kfile PATH]\n";
     return 2;
   }
-  std::string file = argv[1], mode = "host", kfile;
+  std::string file = argv[1], mode = "host", kfile, binprefix;
   size_t parts = static_cast<size_t>(omp_get_num_procs());
   for (int i = 2; i + 1 < argc; i += 2) {
     std::string a = argv[i];
     if (a == "--mode") mode = argv[i + 1];
     else if (a == "--parts") parts = std::stoul(argv[i + 1]);
     else if (a == "--kfile") kfile = argv[i + 1];
+    else if (a == "--bin") binprefix = argv[i + 1];
   }
   try {
     auto t0 = std::chrono::steady_clock::now();
@@ -38,6 +41,11 @@ int main(int argc, char** argv) {
       std::ofstream out(kfile);
       for (size_t i = 0; i < r.T; ++i)
         out << '(' << r.tris[i][0] << ',' << r.tris[i][1] << ',' << r.tris[i][2] << ")\t" << r.k[i] << '\n';
+    }
+    if (!binprefix.empty()) {
+      std::ofstream fk(binprefix + ".k", std::ios::binary), ft(binprefix + ".tri", std::ios::binary);
+      fk.write(reinterpret_cast<const char*>(r.k.data()), static_cast<std::streamsize>(r.k.size() * 4));
+      ft.write(reinterpret_cast<const char*>(r.tris.data()), static_cast<std::streamsize>(r.tris.size() * 12));
     }
     std::printf(
         "{\"mode\": \"%s\", \"n\": %u, \"m\": %zu, \"T\": %llu, \"Q\": %llu, \"K_max\": %d, "
