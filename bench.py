@@ -99,6 +99,38 @@ def run_cpu_baseline(tmp: str, threads: int) -> dict:
                       f"{'reference host path compiled from cliques.hpp+bucket_ex.hpp (serial pstl: no TBB)' if kind == 'reference' else 'oracle restatement'}"}
 
 
+def run_batched(nuc34, count: int, seed: int = 3) -> dict:
+    """Many-graphs workload: `count` graphs of the seeded mixed batch, decomposed
+    as one disjoint union (one pass of every kernel, shared peel rounds) versus one
+    decomposition per graph; both include upload and device orientation. Every
+    graph's slice of the batched result must equal its own decomposition."""
+    gs = [nuc34.gen_batch_graph(seed, i) for i in range(count)]
+    with nuc34.Context(0) as ctx:
+        ctx.load_batch(gs[:8])  # warm-up of both paths (allocator cache, code objects)
+        ctx.decompose("auto")
+        ctx.load_graph(gs[0])
+        ctx.decompose("auto")
+        t0 = time.perf_counter()
+        ctx.load_batch(gs)
+        stb = ctx.decompose("auto")
+        t1 = time.perf_counter()
+        k_b = ctx.copy_k()
+        infos = [ctx.batch_graph(g) for g in range(count)]
+        t2 = time.perf_counter()
+        ks = []
+        for g in gs:
+            ctx.load_graph(g)
+            ctx.decompose("auto")
+            ks.append(ctx.copy_k())
+        t3 = time.perf_counter()
+    plain_s = t3 - t2
+    exact = all(np.array_equal(k_b[i["tri_first"]:i["tri_first"] + i["tri_count"]], kk) for i, kk in zip(infos, ks))
+    return {"graphs": count, "seed": seed, "vertices": int(sum(g.n for g in gs)), "edges": int(sum(g.m for g in gs) // 2),
+            "triangles": stb["triangles"], "four_cliques": stb["four_cliques"], "rounds": stb["rounds"],
+            "batched_s": t1 - t0, "plain_s": plain_s, "graphs_per_s_batched": count / (t1 - t0),
+            "graphs_per_s_plain": count / plain_s, "speedup": plain_s / (t1 - t0), "bit_exact_vs_plain": bool(exact)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +139,8 @@ def main():
     ap.add_argument("--workload", default="rmat16", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain comparison")
+    ap.add_argument("--batch-graphs", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (graphs shard by graph or by "
@@ -133,6 +167,7 @@ def main():
             steps.append(ctx.decompose(wl["variant"], wl["parts"]))
         t1 = time.perf_counter()
         ctx.close()
+        batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
         cpu = None if args.no_cpu_baseline else run_cpu_baseline(tmp, args.cpu_threads)
 
     st = steps[-1]
@@ -177,6 +212,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(rec))
 

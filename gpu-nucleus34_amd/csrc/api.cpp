@@ -11,7 +11,9 @@
 #include <vector>
 
 #include "../../include/nuc34.h"
+#include "../host/gen.hpp"
 #include "../host/graph_io.hpp"
+#include "../host/presets.hpp"
 #include "cliques.hpp"
 #include "kernels.hpp"
 #include "scan.hpp"
@@ -34,6 +36,8 @@ struct nuc34_ctx {
   uint32_t n = 0;
   uint64_t m = 0;
   DevBuf O, E, src;
+  // batch: vertex and oriented-edge boundaries of the graphs of a union
+  std::vector<uint64_t> batch_v, batch_e;
   // triangles and the bit-packed incidence
   bool has_tris = false;
   uint64_t T = 0;
@@ -546,6 +550,48 @@ int nuc34_graph_orient_host(nuc34_graph* g) {
 
 void nuc34_graph_free(nuc34_graph* g) { delete g; }
 
+namespace {
+int graph_from_generated(const gen::Edges& e, nuc34_graph** out) {
+  std::vector<uint32_t> s(e.size()), d(e.size());
+  for (size_t i = 0; i < e.size(); ++i) {
+    s[i] = e[i].first;
+    d[i] = e[i].second;
+  }
+  auto* g = new nuc34_graph;
+  g->lg.g = csr_from_pairs(s.data(), d.data(), s.size());
+  g->lg.lines = e.size();
+  *out = g;
+  return NUC34_OK;
+}
+}  // namespace
+
+int nuc34_gen_preset(const char* name, nuc34_graph** out) {
+  try {
+    return graph_from_generated(preset_edges(name), out);
+  } catch (const std::exception&) {
+    return NUC34_E_INVALID;
+  }
+}
+
+int nuc34_gen_batch_graph(uint64_t seed, uint64_t index, nuc34_graph** out) {
+  try {
+    return graph_from_generated(gen::batch_graph(seed, index, kBatchNLo, kBatchNHi), out);
+  } catch (const std::exception&) {
+    return NUC34_E_INVALID;
+  }
+}
+
+int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   uint32_t planted_cliques, uint32_t planted_size, nuc34_graph** out) {
+  try {
+    auto e = gen::rmat(scale, edge_factor, a, b, c, seed);
+    if (planted_cliques) add_planted(e, scale, seed, planted_cliques, planted_size);
+    return graph_from_generated(e, out);
+  } catch (const std::exception&) {
+    return NUC34_E_INVALID;
+  }
+}
+
 int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
   auto* c = new nuc34_ctx;
   c->device = device;
@@ -584,6 +630,10 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
 const char* nuc34_last_error(const nuc34_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int oriented) {
+  if (c) {
+    c->batch_v.clear();
+    c->batch_e.clear();
+  }
   if (!c || !O || (m && !E)) return fail(c, NUC34_E_INVALID, "null argument");
   if (O[0] != 0 || O[n] != m) return fail(c, NUC34_E_INVALID, "offsets do not span the edge array");
   if (m >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "more than 2^32-1 edges");
@@ -628,6 +678,62 @@ int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t 
     c->has_graph = true;
     c->st.n = n;
     c->st.m = c->m;
+    return NUC34_OK;
+  });
+}
+
+int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
+                     const uint32_t* E, int oriented) {
+  if (!c || !n || !m || !O || (count && !E)) return fail(c, NUC34_E_INVALID, "null argument");
+  std::vector<uint32_t> Ou, Eu;
+  std::vector<uint64_t> vfirst(count + 1, 0), efirst_in(count + 1, 0);
+  for (uint32_t g = 0; g < count; ++g) {
+    vfirst[g + 1] = vfirst[g] + n[g];
+    efirst_in[g + 1] = efirst_in[g] + m[g];
+  }
+  if (vfirst[count] >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 vertices");
+  if (efirst_in[count] >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 edges");
+  Ou.resize(vfirst[count] + 1);
+  Eu.resize(efirst_in[count]);
+  const uint32_t* Og = O;
+  for (uint32_t g = 0; g < count; ++g) {
+    if (Og[0] != 0 || Og[n[g]] != m[g]) return fail(c, NUC34_E_INVALID, "graph " + std::to_string(g) + ": offsets do not span its edges");
+    for (uint32_t v = 0; v < n[g]; ++v) Ou[vfirst[g] + v] = static_cast<uint32_t>(efirst_in[g] + Og[v]);
+    const uint32_t vb = static_cast<uint32_t>(vfirst[g]);
+    for (uint64_t i = 0; i < m[g]; ++i) Eu[efirst_in[g] + i] = E[efirst_in[g] + i] + vb;
+    Og += n[g] + 1;
+  }
+  Ou[vfirst[count]] = static_cast<uint32_t>(efirst_in[count]);
+  const int rc = nuc34_load_csr(c, Ou.data(), Eu.empty() ? Ou.data() : Eu.data(),
+                                static_cast<uint32_t>(vfirst[count]), efirst_in[count], oriented);
+  if (rc != NUC34_OK) return rc;
+  return guard(c, [&]() -> int {
+    // oriented edge boundaries: the union's oriented offsets at each graph's first vertex
+    std::vector<uint32_t> Oo(vfirst[count] + 1);
+    NUC34_HIP(hipMemcpyAsync(Oo.data(), c->O.as<void>(), Oo.size() * 4, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    c->batch_v = vfirst;
+    c->batch_e.resize(count + 1);
+    for (uint32_t g = 0; g <= count; ++g) c->batch_e[g] = Oo[vfirst[g]];
+    return NUC34_OK;
+  });
+}
+
+int nuc34_batch_graph(nuc34_ctx* c, uint32_t g, uint64_t* vertex_first, uint64_t* edge_first, uint64_t* tri_first,
+                      uint64_t* tri_count) {
+  if (!c || c->batch_v.empty()) return fail(c, NUC34_E_STATE, "no batch loaded");
+  if (g + 1 >= c->batch_v.size()) return fail(c, NUC34_E_INVALID, "graph index out of range");
+  if (vertex_first) *vertex_first = c->batch_v[g];
+  if (edge_first) *edge_first = c->batch_e[g];
+  if (!tri_first && !tri_count) return NUC34_OK;
+  if (!c->has_tris) return fail(c, NUC34_E_STATE, "triangles not computed");
+  return guard(c, [&]() -> int {
+    uint64_t lo = 0, hi = 0;
+    NUC34_HIP(hipMemcpyAsync(&lo, c->offA.as<uint64_t>() + c->batch_e[g], 8, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipMemcpyAsync(&hi, c->offA.as<uint64_t>() + c->batch_e[g + 1], 8, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    if (tri_first) *tri_first = lo;
+    if (tri_count) *tri_count = hi - lo;
     return NUC34_OK;
   });
 }

@@ -63,6 +63,10 @@ def _load() -> C.CDLL:
         "nuc34_graph_copy": (C.c_int, [vp, u32p, u32p]),
         "nuc34_graph_orient_host": (C.c_int, [vp]),
         "nuc34_graph_free": (None, [vp]),
+        "nuc34_gen_preset": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "nuc34_gen_batch_graph": (C.c_int, [C.c_uint64, C.c_uint64, C.POINTER(vp)]),
+        "nuc34_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                     C.c_uint32, C.c_uint32, C.POINTER(vp)]),
         "nuc34_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
         "nuc34_ctx_destroy": (None, [vp]),
         "nuc34_last_error": (C.c_char_p, [vp]),
@@ -78,6 +82,8 @@ def _load() -> C.CDLL:
         "nuc34_copy_fourcliques": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p]),
         "nuc34_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
         "nuc34_reset": (C.c_int, [vp]),
+        "nuc34_load_batch": (C.c_int, [vp, C.c_uint32, u32p, u64p, u32p, u32p, C.c_int]),
+        "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
@@ -149,6 +155,30 @@ def graph_from_pairs(src, dst) -> HostGraph:
     return _take_graph(h)
 
 
+def gen_preset(name: str) -> HostGraph:
+    h = C.c_void_p()
+    rc = _lib.nuc34_gen_preset(name.encode(), C.byref(h))
+    if rc != OK:
+        raise Nuc34Error(rc, f"unknown preset {name}")
+    return _take_graph(h)
+
+
+def gen_batch_graph(seed: int, index: int) -> HostGraph:
+    h = C.c_void_p()
+    rc = _lib.nuc34_gen_batch_graph(seed, index, C.byref(h))
+    if rc != OK:
+        raise Nuc34Error(rc, "batch generator failed")
+    return _take_graph(h)
+
+
+def gen_rmat(scale, edge_factor, a, b, c, seed, planted_cliques=0, planted_size=0) -> HostGraph:
+    h = C.c_void_p()
+    rc = _lib.nuc34_gen_rmat(scale, edge_factor, a, b, c, seed, planted_cliques, planted_size, C.byref(h))
+    if rc != OK:
+        raise Nuc34Error(rc, "rmat generator failed")
+    return _take_graph(h)
+
+
 def device_info(device: int = 0) -> dict:
     buf = C.create_string_buffer(4096)
     rc = _lib.nuc34_device_info(device, buf, 4096)
@@ -195,6 +225,31 @@ class Context:
 
     def load_graph(self, g: HostGraph):
         self.load_csr(g.O, g.E, g.oriented)
+
+    def load_batch(self, graphs, oriented: bool = False):
+        """Load many graphs (HostGraph or (O, E) pairs) as one disjoint union."""
+        Os, Es, ns, ms = [], [], [], []
+        for g in graphs:
+            O, E = (g.O, g.E) if isinstance(g, HostGraph) else g
+            Os.append(np.ascontiguousarray(O, dtype=np.uint32))
+            Es.append(np.ascontiguousarray(E, dtype=np.uint32))
+            ns.append(len(O) - 1)
+            ms.append(len(E))
+        O = np.concatenate(Os) if Os else np.zeros(1, np.uint32)
+        E = np.concatenate(Es) if Es and sum(ms) else np.zeros(1, np.uint32)
+        n = np.array(ns, dtype=np.uint32)
+        m = np.array(ms, dtype=np.uint64)
+        self._check(_lib.nuc34_load_batch(self._h, len(ns), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          _u32(O), _u32(E), int(bool(oriented))))
+        self._batch = len(ns)
+
+    def batch_graph(self, g: int, with_triangles: bool = True) -> dict:
+        vf, ef, tf, tc = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        if with_triangles:
+            self._check(_lib.nuc34_batch_graph(self._h, g, C.byref(vf), C.byref(ef), C.byref(tf), C.byref(tc)))
+        else:
+            self._check(_lib.nuc34_batch_graph(self._h, g, C.byref(vf), C.byref(ef), None, None))
+        return {"vertex_first": vf.value, "edge_first": ef.value, "tri_first": tf.value, "tri_count": tc.value}
 
     def dims(self):
         n, m = C.c_uint32(), C.c_uint64()

@@ -96,6 +96,17 @@ int nuc34_graph_copy(const nuc34_graph* g, uint32_t* O, uint32_t* E);
 int nuc34_graph_orient_host(nuc34_graph* g);
 void nuc34_graph_free(nuc34_graph* g);
 
+/* Seeded synthetic datasets (undirected, as if loaded from the generator's
+ * .edges file). No counterpart in the reference, which downloads datasets:
+ * named workloads "er1000", "rmat16", "rmat18", "rmat20k30", and graph
+ * `index` of the mixed many-graphs batch drawn from `seed` (n log-uniform in
+ * [200, 50000]; G(n,M) with mean degree 8-64, Barabasi-Albert m 4-16, or
+ * G(n,M) degree 8 with planted 10-40 cliques, by index mod 3). */
+int nuc34_gen_preset(const char* name, nuc34_graph** out);
+int nuc34_gen_batch_graph(uint64_t seed, uint64_t index, nuc34_graph** out);
+int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   uint32_t planted_cliques, uint32_t planted_size, nuc34_graph** out);
+
 /* ---- device context ------------------------------------------------------ */
 
 /* hbm_budget: bytes the context may hold on the device (0 = no limit). */
@@ -108,6 +119,22 @@ const char* nuc34_last_error(const nuc34_ctx* ctx);
  * given (lists sorted, acyclic). Resets every later stage. */
 int nuc34_load_csr(nuc34_ctx* ctx, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int oriented);
 int nuc34_graph_dims(const nuc34_ctx* ctx, uint32_t* n, uint64_t* m_oriented);
+
+/* Many independent graphs as one batch: graph g has n[g] vertices and m[g]
+ * edges; O holds the graphs' offset arrays back to back (n[g]+1 local offsets
+ * each, starting at 0) and E their edge arrays (local vertex ids). They are
+ * loaded as one disjoint union, so every later stage and every peel round
+ * covers all graphs at once; results are per-graph slices. No counterpart in
+ * the reference (which runs one graph per process): the batched mode of the
+ * many-graphs workload. */
+int nuc34_load_batch(nuc34_ctx* ctx, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
+                     const uint32_t* E, int oriented);
+/* Graph g of the loaded batch: its first vertex and first oriented edge in the
+ * union and, once triangles are computed, its triangle slice [tri_first,
+ * tri_first + tri_count) of the union's (a,b)-ordered list. The union's edge
+ * indices of graph g minus edge_first are the graph's own. */
+int nuc34_batch_graph(nuc34_ctx* ctx, uint32_t g, uint64_t* vertex_first, uint64_t* edge_first, uint64_t* tri_first,
+                      uint64_t* tri_count);
 int nuc34_copy_oriented(nuc34_ctx* ctx, uint32_t* O, uint32_t* E);
 
 /* Triangles in the reference's canonical (a,b) order; `parts` >= 1 splits the

@@ -220,3 +220,31 @@ def test_k4_list(name):
         p_, q_, r_, s_ = vs
         assert [verts(int(t)) for t in q] == [{p_, q_, s_}, {p_, q_, r_}, {p_, r_, s_}, {q_, r_, s_}]
     assert all(quds[i][1] <= quds[i + 1][1] for i in range(len(quds) - 1))
+
+
+@pytest.mark.parametrize("variant", ["direct", "ondemand"])
+def test_batch_matches_single_graphs(variant):
+    """A batch is decomposed as one disjoint union: every graph's slice must
+    equal its own decomposition, and the smallest ones the oracle's."""
+    idx = list(range(9)) + [30, 31, 32]
+    gs = [nuc34.gen_batch_graph(3, i) for i in idx]
+    with nuc34.Context() as ctx:
+        ctx.load_batch(gs)
+        st = ctx.decompose(variant)
+        tris = ctx.copy_triangles()
+        k = ctx.copy_k()
+        sup = ctx.copy_support()
+        infos = [ctx.batch_graph(g) for g in range(len(gs))]
+    assert st["unpeeled"] == 0
+    assert sum(i["tri_count"] for i in infos) == len(k)
+    order = sorted(range(len(gs)), key=lambda j: gs[j].m)
+    for j, g in enumerate(gs):
+        info = infos[j]
+        sl = slice(info["tri_first"], info["tri_first"] + info["tri_count"])
+        t_single, s_single, k_single, st_single = gpu_run(g, variant)
+        assert np.array_equal(tris[sl] - np.uint32(info["edge_first"]), t_single), j
+        assert np.array_equal(k[sl], k_single), j
+        assert np.array_equal(sup[sl], s_single), j
+        if j in order[:3]:
+            ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("host", 4)
+            assert np.array_equal(k_single, ref["k"]) and np.array_equal(t_single, ref["tris"]), j
