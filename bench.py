@@ -14,6 +14,11 @@ value = four-cliques / second = Q / (mean step time).
 Prints one JSON line with the driver's contract, plus:
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average launch time (HIP events on the context stream);
+                traffic = its HBM bytes per launch from two rocprofv3 --pmc
+                child runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
+                size in the same process;
+  batched       512 graphs of the seeded mixed batch decomposed as one disjoint
+                union versus one decomposition per graph (graphs/s, bit-exact);
   cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
                 the reference headers) timed on this host's cores on a bounded
                 sample (R-MAT scale 14, same generator parameters).
@@ -75,6 +80,61 @@ def algorithmic_bytes(kernel: str, g: dict, st: dict) -> float:
     if kernel == "peel_rounds":
         return 4 * Q * 12 + 3 * Q * 8 + T * 20  # every list once, debits, frontier + offsets
     return 0.0
+
+
+# Kernels that make up each timed group (the per-kernel event timers of the
+# library time these groups; the peel group is one captured graph of rounds).
+GROUP_KERNELS = {
+    "tri_enum_bits": ("k_tri_enum<false>",),
+    "tri_enum_emit": ("k_tri_enum<true>",),
+    "support_bits": ("k_support_bits",),
+    "k4_owned": ("k_k4_owned",),
+    "k4_place": ("k_k4_place",),
+    "peel_rounds": ("k_peel_level", "k_peel_collect", "k_peel_debit", "k_peel_finish"),
+}
+
+
+def _pmc_rows(db_path: str):
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select name, counter_value from pmc_events order by dispatch_id").fetchall()
+    con.close()
+    return [(n.replace("(anonymous namespace)::", "").split("(")[0], v) for n, v in rows]
+
+
+def run_traffic(workload: str, group: str, launches: float, tmp: str, calib_mib: int = 1024) -> dict | None:
+    """HBM bytes per launch of `group` from two rocprofv3 --pmc runs of a
+    one-step bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
+    by what the same counter reads for a device-to-device copy of `calib_mib` MiB
+    made first in the same process. None when rocprofv3 is unavailable or fails."""
+    prof = "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-batch",
+             "--no-cpu-baseline", "--no-traffic", "--workload", workload, "--copy-calibration-mib", str(calib_mib)]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(tmp, counter)
+        try:
+            subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "--", *child], check=True, timeout=240,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        except (subprocess.SubprocessError, OSError):
+            return None
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("results.db")]
+        if not dbs:
+            return None
+        rows = _pmc_rows(dbs[0])
+        copies = [v for n, v in rows if "copyBuffer" in n]
+        if not copies:
+            return None
+        scale = calib_mib * 1024.0 / max(copies)  # counter KiB -> true KiB
+        names = GROUP_KERNELS.get(group, ())
+        kib = sum(v for n, v in rows if any(k in n for k in names))
+        res[counter] = (kib * scale * 1024.0 / launches, scale)
+    return {"bytes_per_launch": res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0],
+            "fetch_bytes_per_launch": res["FETCH_SIZE"][0], "write_bytes_per_launch": res["WRITE_SIZE"][0],
+            "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1],
+            "calibration": f"{calib_mib} MiB device-to-device copy in the profiled process"}
 
 
 def run_cpu_baseline(tmp: str, threads: int) -> dict:
@@ -141,6 +201,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain comparison")
     ap.add_argument("--batch-graphs", type=int, default=512)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first make a device-to-device copy of this many MiB (counter calibration under --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (graphs shard by graph or by "
@@ -148,6 +211,12 @@ def main():
         sys.exit(2)
     import nuc34
 
+    if args.copy_calibration_mib:
+        import torch
+        a = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        del a
     wl = WORKLOADS[args.workload]
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "graph.edges")
@@ -169,20 +238,20 @@ def main():
         ctx.close()
         batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
         cpu = None if args.no_cpu_baseline else run_cpu_baseline(tmp, args.cpu_threads)
+        st = steps[-1]
+        kern = {}
+        for s_ in steps:
+            for name, k in s_["kernels"].items():
+                a = kern.setdefault(name, {"ms": 0.0, "launches": 0})
+                a["ms"] += k["ms"]
+                a["launches"] += k["launches"]
+        dom = max(kern, key=lambda k: kern[k]["ms"])
+        launches_per_step = kern[dom]["launches"] / args.steps
+        traffic = None if args.no_traffic else run_traffic(args.workload, dom, launches_per_step, tmp)
 
-    st = steps[-1]
     ms = (t1 - t0) / args.steps * 1e3
     value = st["four_cliques"] / (ms / 1e3)
-    # dominant kernel over the timed steps
-    kern = {}
-    for s_ in steps:
-        for name, k in s_["kernels"].items():
-            a = kern.setdefault(name, {"ms": 0.0, "launches": 0})
-            a["ms"] += k["ms"]
-            a["launches"] += k["launches"]
-    dom = max(kern, key=lambda k: kern[k]["ms"])
     per_launch_ms = kern[dom]["ms"] / kern[dom]["launches"]
-    launches_per_step = kern[dom]["launches"] / args.steps
     bytes_per_launch = algorithmic_bytes(dom, gs, st) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
     golden = json.load(open(os.path.join(ROOT, "tests", "golden", "hashes.json"))).get(wl["gen"][-1], {})
@@ -209,7 +278,9 @@ def main():
                                                            "ms_incidence", "ms_peel", "ms_total")},
                    "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_detail": traffic,
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms},
         "cpu_baseline": cpu,
         "batched": batched,

@@ -404,22 +404,26 @@ void run_peel(nuc34_ctx* c, int variant) {
     NUC34_HIP(hipStreamSynchronize(c->s));
     return;
   }
-  DevBuf dd = c->alloc.get(T * 4, "peel supports");
-  DevBuf rnd = c->alloc.get(T * 4, "death rounds");
+  const uint64_t words = (T + 15) / 16;
+  DevBuf dd = c->alloc.get(words * 64, "peel supports");
+  DevBuf stw = c->alloc.get(words * 4, "peel states");
   DevBuf b0 = c->alloc.get(T * 4, "frontier");
   DevBuf b1 = c->alloc.get(T * 4, "frontier");
-  DevBuf a0 = c->alloc.get(T * 4, "alive list");
-  DevBuf a1 = c->alloc.get(T * 4, "alive list");
-  DevBuf ctlb = c->alloc.get(sizeof(PeelCtl), "peel control");
-  PeelBufs pb{dd.as<int32_t>(), rnd.as<uint32_t>(), c->kv.as<int32_t>(), b0.as<uint32_t>(), b1.as<uint32_t>(), a0.as<uint32_t>(),
-              a1.as<uint32_t>(), variant == NUC34_DIRECT ? nullptr : c->bigq.as<uint32_t>(), ctlb.as<PeelCtl>(), T};
+  DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
+  PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
+              b0.as<uint32_t>(), b1.as<uint32_t>(),  variant == NUC34_DIRECT ? nullptr : c->bigq.as<uint32_t>(),
+              ctlb.as<PeelCtl>(), T};
   const TriView v = triview(c);
+  static_assert(kRoundsPerLaunch % 2 == 0, "round parity must equal slot parity");
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {
-      if (variant == NUC34_DIRECT) launch_peel_round_direct(pb, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s);
-      else launch_peel_round_ondemand(pb, v, c->s);
+      if (variant == NUC34_DIRECT)
+        launch_peel_round_direct(pb, r, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s);
+      else
+        launch_peel_round_ondemand(pb, r, v, c->s);
     }
   };
+  PeelCtl hs[2] = {};
   PeelCtl h{};
   c->st.ms_peel = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
@@ -441,8 +445,11 @@ void run_peel(nuc34_ctx* c, int variant) {
         if (exec) NUC34_HIP(hipGraphLaunch(exec, c->s));
         else enqueue_rounds(kRoundsPerLaunch);
       });
-      NUC34_HIP(hipMemcpyAsync(&h, ctlb.as<void>(), sizeof(h), hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipMemcpyAsync(hs, ctlb.as<void>(), sizeof(hs), hipMemcpyDeviceToHost, c->s));
       NUC34_HIP(hipStreamSynchronize(c->s));
+      // the round that finished writes done into both slots; its successor's
+      // slot (the higher round) holds the final counts
+      h = hs[0].round >= hs[1].round ? hs[0] : hs[1];
       if (h.done) break;
       if (l > max_launches) throw std::logic_error("peel did not finish");
     }

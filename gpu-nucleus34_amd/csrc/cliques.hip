@@ -319,133 +319,147 @@ __global__ void k_k4_quads(const uint64_t* __restrict__ k4, uint64_t first, uint
 
 // ---- peel ------------------------------------------------------------------
 
-constexpr int32_t kDead = 0x7FFFFFFF;
-constexpr uint32_t kAliveRound = 0xFFFFFFFFu;
+constexpr uint32_t kAlive = 0, kCur = 1, kNext = 2, kGone = 3;
+constexpr uint32_t kNoMin = 0xFFFFFFFFu;
 
-// Level in force for this round: when the frontier ran empty, the new level is
-// max(level, min alive support) from k_peel_level (committed by k_peel_finish).
+__device__ __forceinline__ uint32_t tri_state(const uint32_t* st, uint32_t t) {
+  return (st[t >> 4] >> ((t & 15u) * 2u)) & 3u;
+}
+
+// Start of a round for 16 fields at once: kCur -> kGone, kNext -> kCur, kAlive
+// and kGone unchanged (new high bit = old low bit, new low bit = low | high).
+__device__ __forceinline__ uint32_t advance_states(uint32_t w) {
+  const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
+  return (lo << 1) | lo | hi;
+}
+
+// Low bit of every kAlive field of w.
+__device__ __forceinline__ uint32_t alive_fields(uint32_t w) { return ~(w | (w >> 1)) & 0x55555555u; }
+
+// The 16 supports of state word i (dd is padded to whole words).
+__device__ __forceinline__ void load_word_dd(const int32_t* dd, uint64_t i, int32_t d[16]) {
+  const int4* d4 = reinterpret_cast<const int4*>(dd) + 4 * i;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int4 x = d4[k];
+    d[4 * k] = x.x;
+    d[4 * k + 1] = x.y;
+    d[4 * k + 2] = x.z;
+    d[4 * k + 3] = x.w;
+  }
+}
+
+// Level in force for the round controlled by c: when the frontier ran empty,
+// max(level, min alive support).
 __device__ __forceinline__ int32_t round_level(const PeelCtl* c) {
   if (!c->need_level) return c->level;
   const uint32_t m = c->minval;
-  return (m != 0xFFFFFFFFu && static_cast<int32_t>(m) > c->level) ? static_cast<int32_t>(m) : c->level;
+  return (m != kNoMin && static_cast<int32_t>(m) > c->level) ? static_cast<int32_t>(m) : c->level;
 }
 
-__global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t* __restrict__ sup) {
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < p.T;
-       t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    p.dd[t] = sup[t];
-    p.rnd[t] = kAliveRound;
-    p.kv[t] = -1;
-    p.alive0[t] = static_cast<uint32_t>(t);
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    PeelCtl* c = p.ctl;
-    c->count[0] = c->count[1] = 0;
-    c->parity = 0;
-    c->need_level = 1;
-    c->minval = 0xFFFFFFFFu;
-    c->level = 0;
-    c->round = 0;
-    c->done = 0;
-    c->peeled = 0;
-    c->big = 0;
-    c->error = 0;
-    c->nalive[0] = static_cast<uint32_t>(p.T);
-    c->nalive[1] = 0;
-    c->apar = 0;
-  }
-}
-
-// Level selection, step 1 (only when the frontier ran empty): min support over
-// the alive list; one atomic per workgroup.
-__global__ __launch_bounds__(kBlock) void k_peel_level(PeelBufs p) {
-  const PeelCtl* c = p.ctl;
-  if (c->done || !c->need_level) return;
+// Workgroup min of mn into *out, one atomic per workgroup. Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void block_min_to(uint32_t mn, uint32_t* out) {
   __shared__ uint32_t wmin[kWPB];
-  uint32_t mn = 0xFFFFFFFFu;
-  const uint32_t* alive = c->apar ? p.alive1 : p.alive0;
-  const uint32_t na = c->nalive[c->apar];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const int32_t d = p.dd[alive[i]];
-    if (d != kDead) mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
-  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mn = min(mn, static_cast<uint32_t>(__shfl_xor(mn, off)));
   if (lane_id() == 0) wmin[threadIdx.x / kWave] = mn;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kWPB; ++i) mn = min(mn, wmin[i]);
-    if (mn != 0xFFFFFFFFu) atomic_min_u32(&p.ctl->minval, mn);
+    if (mn != kNoMin) atomic_min_u32(out, mn);
   }
 }
 
-// Level selection, step 2: every alive triangle with support <= level dies this
-// round (k = level); the others move to the other alive buffer. Each thread
-// takes kCollectItems list entries per tile, so one pair of atomics reserves
-// the output slots of a whole 4096-entry tile.
-constexpr int kCollectItems = 16;
+__global__ void k_peel_ctl_init(PeelCtl* ctl) {
+  for (int s = 0; s < 2; ++s) {
+    PeelCtl& c = ctl[s];
+    c.ncur = c.nnext = c.sel = 0;
+    c.need_level = s == 0 ? 1u : 0u;
+    c.minval = kNoMin;
+    c.level = 0;
+    c.round = 0;
+    c.done = 0;
+    c.big = 0;
+    c.error = 0;
+    c.peeled = 0;
+  }
+}
 
-__global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p) {
-  PeelCtl* c = p.ctl;
-  if (c->done || !c->need_level) return;
-  __shared__ uint32_t wd[kWPB], wk[kWPB], base[2];
-  const int32_t level = round_level(c);
-  const uint32_t round = c->round;
-  const uint32_t par = c->parity;
-  const uint32_t ap = c->apar;
-  const uint32_t* alive = ap ? p.alive1 : p.alive0;
-  uint32_t* keep = ap ? p.alive0 : p.alive1;
-  const uint32_t na = c->nalive[ap];
-  uint32_t* cur = par ? p.buf1 : p.buf0;
-  const unsigned w = threadIdx.x / kWave;
-  const unsigned lane = lane_id();
-  constexpr uint32_t tile = kBlock * kCollectItems;
-  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * tile; b0 < na; b0 += static_cast<uint64_t>(gridDim.x) * tile) {
-    uint32_t tv[kCollectItems];
-    uint32_t die = 0, stay = 0;  // bit j: item j
+// dd = support, kv = -1, every triangle alive (padding fields kGone), and the
+// first level's minimum into ctl[0].
+__global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t* __restrict__ sup) {
+  const uint64_t W = (p.T + 15) / 16;
+  uint32_t mn = kNoMin;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < W;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t w = 0;
 #pragma unroll
-    for (int j = 0; j < kCollectItems; ++j) {
-      const uint64_t i = b0 + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
-      tv[j] = 0;
-      if (i < na) {
-        const uint32_t t = alive[i];
-        const int32_t d = p.dd[t];
-        tv[j] = t;
-        if (d <= level) die |= 1u << j;
-        else if (d != kDead) stay |= 1u << j;
+    for (uint32_t f = 0; f < 16; ++f) {
+      const uint64_t t = 16 * i + f;
+      if (t < p.T) {
+        const int32_t d = sup[t];
+        p.dd[t] = d;
+        p.kv[t] = -1;
+        mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
+      } else {
+        p.dd[t] = 0;
+        w |= kGone << (2 * f);
       }
     }
-    const uint32_t nd = static_cast<uint32_t>(__popc(die)), nk = static_cast<uint32_t>(__popc(stay));
-    const uint32_t id = wave_inclusive_sum(nd), ik = wave_inclusive_sum(nk);
-    if (lane == kWave - 1) {
-      wd[w] = id;
-      wk[w] = ik;
+    p.st[i] = w;
+  }
+  block_min_to(mn, &p.ctl[0].minval);
+}
+
+// Level change (the frontier ran empty): every alive triangle with support
+// <= level dies this round (k = level). One state word (16 triangles) per
+// thread; one atomic per workgroup tile reserves the frontier slots.
+__global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p, PeelCtl* in) {
+  if (in->done || !in->need_level) return;
+  __shared__ uint32_t wd[kWPB], base;
+  const int32_t level = round_level(in);
+  uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  const uint64_t W = (p.T + 15) / 16;
+  const unsigned wv = threadIdx.x / kWave;
+  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kBlock; b0 < W; b0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    const uint64_t i = b0 + threadIdx.x;
+    uint32_t w = 0, die = 0;
+    if (i < W) {
+      w = p.st[i];
+      const uint32_t al = alive_fields(w);
+      if (al) {
+        int32_t d[16];
+        load_word_dd(p.dd, i, d);
+#pragma unroll
+        for (int f = 0; f < 16; ++f)
+          if (((al >> (2 * f)) & 1u) && d[f] <= level) die |= 1u << f;
+      }
     }
+    const uint32_t nd = static_cast<uint32_t>(__popc(die));
+    const uint32_t id = wave_inclusive_sum(nd);
+    if (lane_id() == kWave - 1) wd[wv] = id;
     __syncthreads();
     if (threadIdx.x == 0) {
-      uint32_t td = 0, tk = 0;
+      uint32_t td = 0;
       for (int q = 0; q < kWPB; ++q) {
-        const uint32_t x = wd[q], y = wk[q];
+        const uint32_t x = wd[q];
         wd[q] = td;
-        wk[q] = tk;
         td += x;
-        tk += y;
       }
-      base[0] = td ? atomic_add_u32(&c->count[par], td) : 0;
-      base[1] = tk ? atomic_add_u32(&c->nalive[ap ^ 1u], tk) : 0;
+      base = td ? atomic_add_u32(&in->ncur, td) : 0;
     }
     __syncthreads();
-    uint32_t od = base[0] + wd[w] + id - nd, ok = base[1] + wk[w] + ik - nk;
-#pragma unroll
-    for (int j = 0; j < kCollectItems; ++j) {
-      if (die & (1u << j)) {
-        p.rnd[tv[j]] = round;
-        p.kv[tv[j]] = level;
-        p.dd[tv[j]] = kDead;
-        cur[od++] = tv[j];
-      } else if (stay & (1u << j)) {
-        keep[ok++] = tv[j];
+    if (die) {
+      uint32_t od = base + wd[wv] + id - nd;
+      for (int f = 0; f < 16; ++f) {
+        if (!((die >> f) & 1u)) continue;
+        const uint32_t t = static_cast<uint32_t>(16 * i + f);
+        p.kv[t] = level;
+        cur[od++] = t;
+        w |= kCur << (2 * f);
       }
+      p.st[i] = w;
     }
     __syncthreads();
   }
@@ -456,66 +470,88 @@ __global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p) {
 // as surviving for ownership; an earlier death means the K4 is gone; a
 // same-round death with a lower index owns it. A survivor falling from
 // level+1 to level is pushed: its k is fixed now and it dies next round.
+// kCur and kGone were set before this kernel started; a push racing with this
+// read can only be seen as kAlive, which gives the same owner, and the extra
+// debit of a pushed triangle cannot push it again (its support is already at
+// most level).
 __device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
-                                           uint32_t round, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
-  uint32_t rq[3];
+                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+  uint32_t sq[3];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    rq[s] = p.rnd[q[s]];
-    if (rq[s] < round) return;
-    if (rq[s] == round && q[s] < t) return;
+    sq[s] = tri_state(p.st, q[s]);
+    if (sq[s] == kGone) return;
+    if (sq[s] == kCur && q[s] < t) return;
   }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    if (rq[s] != kAliveRound) continue;
+    if (sq[s] != kAlive) continue;
     const int32_t old = atomic_sub_i32(&p.dd[q[s]], 1);
     if (old == level + 1) {
-      p.rnd[q[s]] = round + 1;
+      atomic_or_u32(&p.st[q[s] >> 4], kNext << ((q[s] & 15u) * 2u));
       p.kv[q[s]] = level;
-      __hip_atomic_store(&p.dd[q[s]], kDead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       nxt[atomic_add_u32(ncount, 1u)] = q[s];
     }
   }
 }
 
-// End of a round, one workgroup: commit the level, flip the buffers, count the
-// fixed triangles; an empty next frontier triggers level selection.
-__global__ void k_peel_finish(PeelBufs p) {
-  if (threadIdx.x != 0) return;
-  PeelCtl* c = p.ctl;
-  if (c->done) return;
-  if (c->need_level) {
-    c->level = round_level(c);
-    if (c->minval == 0xFFFFFFFFu || c->count[c->parity] == 0) {
-      c->error = c->minval == 0xFFFFFFFFu ? 1u : 2u;  // nothing alive / nothing collected
-      c->done = 1;
-      return;
+// End of a round: advance every state word (this round's dead become kGone,
+// the pushed become the next frontier); when nothing was pushed, the min alive
+// support for the next level goes into out->minval. Workgroup 0 writes the
+// next round's control into `out`; no other workgroup reads `out`.
+__global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out) {
+  if (in->done) return;
+  const bool need = in->nnext == 0;
+  const uint64_t W = (p.T + 15) / 16;
+  uint32_t mn = kNoMin;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < W;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t w = advance_states(p.st[i]);
+    p.st[i] = w;
+    if (need) {
+      const uint32_t al = alive_fields(w);
+      if (al) {
+        int32_t d[16];
+        load_word_dd(p.dd, i, d);
+#pragma unroll
+        for (int f = 0; f < 16; ++f)
+          if ((al >> (2 * f)) & 1u) mn = min(mn, static_cast<uint32_t>(d[f] < 0 ? 0 : d[f]));
+      }
     }
-    c->minval = 0xFFFFFFFFu;
-    c->nalive[c->apar] = 0;
-    c->apar ^= 1u;
   }
-  const uint32_t par = c->parity;
-  c->peeled += c->count[par];
-  c->count[par] = 0;
-  c->parity = par ^ 1u;
-  c->round += 1;
-  c->big = 0;
-  c->need_level = c->count[par ^ 1u] == 0 ? 1u : 0u;
-  if (c->peeled >= p.T) c->done = 1;
+  if (need) block_min_to(mn, &out->minval);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint64_t peeled = in->peeled + in->ncur;
+    uint32_t error = 0, done = 0;
+    if (in->need_level && (in->minval == kNoMin || in->ncur == 0)) {
+      error = in->minval == kNoMin ? 1u : 2u;  // nothing alive / nothing collected
+      done = 1;
+    }
+    if (peeled >= p.T) done = 1;
+    out->ncur = in->nnext;
+    out->nnext = 0;
+    out->sel = in->sel ^ 1u;
+    out->need_level = need ? 1u : 0u;
+    out->level = round_level(in);
+    out->round = in->round + 1;
+    out->big = 0;
+    out->error = error;
+    out->peeled = peeled;
+    out->done = done;
+    in->minval = kNoMin;  // the next round's finish reduces into this slot
+    if (done) in->done = 1;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, const uint64_t* __restrict__ koff,
+__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in,
+                                                             const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
-  const PeelCtl* c = p.ctl;
-  if (c->done) return;
-  const int32_t level = round_level(c);
-  const uint32_t round = c->round;
-  const uint32_t par = c->parity;
-  const uint32_t* cur = par ? p.buf1 : p.buf0;
-  uint32_t* nxt = par ? p.buf0 : p.buf1;
-  uint32_t* ncount = &p.ctl->count[par ^ 1u];
-  const uint32_t n = c->count[par];
+  if (in->done) return;
+  const int32_t level = round_level(in);
+  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
+  uint32_t* ncount = &in->nnext;
+  const uint32_t n = in->ncur;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint32_t nw = gridDim.x * kWPB;
@@ -524,49 +560,44 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, const u
     const uint64_t b = koff[t], e = koff[t + 1];
     for (uint64_t j = b + lane; j < e; j += kWave) {
       const uint32_t* q = lists + 3 * j;
-      const uint32_t qq[3] = {q[0], q[1], q[2]};
-      destroy_k4(t, qq, p, level, round, nxt, ncount);
+      const uint32_t qq[3] = {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
+                              __builtin_nontemporal_load(q + 2)};
+      destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_peel_debit_od_small(PeelBufs p, TriView v) {
-  PeelCtl* c = p.ctl;
-  if (c->done) return;
-  const int32_t level = round_level(c);
-  const uint32_t round = c->round;
-  const uint32_t par = c->parity;
-  const uint32_t* cur = par ? p.buf1 : p.buf0;
-  uint32_t* nxt = par ? p.buf0 : p.buf1;
-  uint32_t* ncount = &c->count[par ^ 1u];
-  const uint32_t n = c->count[par];
+__global__ __launch_bounds__(kBlock) void k_peel_debit_od_small(PeelBufs p, PeelCtl* in, TriView v) {
+  if (in->done) return;
+  const int32_t level = round_level(in);
+  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
+  uint32_t* ncount = &in->nnext;
+  const uint32_t n = in->ncur;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t t = cur[i];
     const TriEdges e = tri_edges(v, t);
     const bool big = tri_len(v, e) > kSmallL;
     bool got;
-    const uint32_t slot = wave_append(&c->big, big, &got);
+    const uint32_t slot = wave_append(&in->big, big, &got);
     if (big) {
       p.bigq[slot] = t;
       continue;
     }
     k4_thread(v, e, [&](uint32_t pb, uint32_t pa, uint32_t pcc) {
       const uint32_t qq[3] = {pb, pa, pcc};
-      destroy_k4(t, qq, p, level, round, nxt, ncount);
+      destroy_k4(t, qq, p, level, nxt, ncount);
     });
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, TriView v) {
+__global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, PeelCtl* in, TriView v) {
   __shared__ uint32_t s1[kWPB][kWave], s2[kWPB][kWave];
-  PeelCtl* c = p.ctl;
-  if (c->done) return;
-  const int32_t level = round_level(c);
-  const uint32_t round = c->round;
-  const uint32_t par = c->parity;
-  uint32_t* nxt = par ? p.buf0 : p.buf1;
-  uint32_t* ncount = &c->count[par ^ 1u];
-  const uint32_t n = c->big;
+  if (in->done) return;
+  const int32_t level = round_level(in);
+  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
+  uint32_t* ncount = &in->nnext;
+  const uint32_t n = in->big;
   const unsigned w = threadIdx.x / kWave;
   const uint32_t nw = gridDim.x * kWPB;
   for (uint32_t q = blockIdx.x * kWPB + w; q < n; q += nw) {
@@ -575,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, TriVie
     k4_wave(v, e, s1[w], s2[w], [&](bool match, uint64_t i0, uint64_t i1, uint64_t i2) {
       if (match) {
         const uint32_t qq[3] = {v.tri[i0], v.tri[i1], v.tri[i2]};
-        destroy_k4(t, qq, p, level, round, nxt, ncount);
+        destroy_k4(t, qq, p, level, nxt, ncount);
       }
     });
   }
@@ -633,24 +664,28 @@ void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, co
 constexpr unsigned kScanGrid = 512;
 
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel_init, dim3(grid_for(p.T ? p.T : 1, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, p.ctl);
+  hipLaunchKernelGGL(k_peel_init, dim3(grid_for((p.T + 15) / 16, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_peel_round_direct(const PeelBufs& p, const uint64_t* koff, const uint32_t* lists, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel_level, dim3(kScanGrid), dim3(kBlock), 0, s, p);
-  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p);
-  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, koff, lists);
-  hipLaunchKernelGGL(k_peel_finish, dim3(1), dim3(kWave), 0, s, p);
+void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
+                              hipStream_t s) {
+  PeelCtl* in = p.ctl + (slot & 1);
+  PeelCtl* out = p.ctl + ((slot + 1) & 1);
+  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
+  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, koff, lists);
+  hipLaunchKernelGGL(k_peel_finish, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel_level, dim3(kScanGrid), dim3(kBlock), 0, s, p);
-  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p);
-  hipLaunchKernelGGL(k_peel_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
-  hipLaunchKernelGGL(k_peel_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, p, v);
-  hipLaunchKernelGGL(k_peel_finish, dim3(1), dim3(kWave), 0, s, p);
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const TriView& v, hipStream_t s) {
+  PeelCtl* in = p.ctl + (slot & 1);
+  PeelCtl* out = p.ctl + ((slot + 1) & 1);
+  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
+  hipLaunchKernelGGL(k_peel_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, v);
+  hipLaunchKernelGGL(k_peel_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, v);
+  hipLaunchKernelGGL(k_peel_finish, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -49,44 +49,52 @@ void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, co
 
 // ---- peel ------------------------------------------------------------------
 //
-// Per-triangle peel state: dd = current support (INT32_MAX once the triangle's
-// k is fixed), rnd = round of death (all ones while alive), kv = k (-1 while
-// alive; the result).
-// Round control lives on the device so rounds run back to back without host
-// round trips; a one-workgroup kernel closes each round.
+// Per-triangle peel state: dd = current support, kv = k (-1 until fixed; the
+// result) and a 2-bit state, 16 triangles per 32-bit word of st:
+//   0 alive: k not fixed yet
+//   1 dies in the current round
+//   2 k fixed in the current round (pushed); dies in the next one
+//   3 died in an earlier round (also the padding fields past T)
+// The ownership test of a K4 reads only these states (4 MB per 16 M
+// triangles), so its random reads stay in L2.
+//
+// Round control lives on the device, so rounds run back to back without host
+// round trips. Round r reads ctl[r & 1] and its last kernel writes the next
+// round's control into ctl[(r + 1) & 1], so no kernel writes a control field
+// that another workgroup of the same kernel reads. Rounds are enqueued in
+// groups of even size, so a round's parity is its slot parity in the group.
 struct PeelCtl {
-  uint32_t count[2];    // sizes of the two frontier buffers
-  uint32_t parity;      // buffer holding the current frontier
-  uint32_t need_level;  // frontier empty: pick the next level
-  uint32_t minval;      // min alive support (level selection)
+  uint32_t ncur;        // size of this round's frontier
+  uint32_t nnext;       // size of the next round's frontier (the debit appends)
+  uint32_t sel;         // buffer holding this round's frontier
+  uint32_t need_level;  // frontier empty: this round picks the next level
+  uint32_t minval;      // min support over alive triangles (when need_level)
   int32_t level;
   uint32_t round;
   uint32_t done;
-  uint64_t peeled;
   uint32_t big;         // frontier triangles queued for the wave path (on-demand)
   uint32_t error;       // nonzero: inconsistent state detected
-  uint32_t nalive[2];   // sizes of the two alive-list buffers
-  uint32_t apar;        // buffer holding the current alive list
+  uint64_t peeled;
 };
 
 struct PeelBufs {
-  int32_t* dd;
-  uint32_t* rnd;
+  int32_t* dd;      // padded to a multiple of 16 entries
+  uint32_t* st;     // ceil(T / 16) state words
   int32_t* kv;
-  uint32_t* buf0;
+  uint32_t* buf0;   // frontier buffers, T entries each
   uint32_t* buf1;
-  uint32_t* alive0;  // triangles not yet fixed at the last level change
-  uint32_t* alive1;
-  uint32_t* bigq;
-  PeelCtl* ctl;
+  uint32_t* bigq;   // on-demand: frontier triangles for the wave path
+  PeelCtl* ctl;     // two slots
   uint64_t T;
 };
 
-// One peel round: level selection (only when the frontier is empty), then the
-// debit of every K4 owned by a dying triangle. Every kernel returns at once
-// when ctl->done is set, so extra rounds are no-ops.
-void launch_peel_round_direct(const PeelBufs& p, const uint64_t* koff, const uint32_t* lists, hipStream_t s);
-void launch_peel_round_ondemand(const PeelBufs& p, const TriView& v, hipStream_t s);
+// One peel round at position `slot` of its group: level selection when the
+// frontier is empty, the debit of every K4 owned by a dying triangle, and the
+// state advance. Every kernel returns at once when the round's control says
+// done, so extra rounds are no-ops.
+void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
+                              hipStream_t s);
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const TriView& v, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
 
 
