@@ -745,6 +745,25 @@ int nuc34_batch_graph(nuc34_ctx* c, uint32_t g, uint64_t* vertex_first, uint64_t
   });
 }
 
+int nuc34_batch_slices(nuc34_ctx* c, uint64_t* edge_first, uint64_t* tri_first) {
+  if (!c || c->batch_v.empty()) return fail(c, NUC34_E_STATE, "no batch loaded");
+  const uint64_t k = c->batch_e.size();  // graphs + 1
+  if (edge_first) std::copy(c->batch_e.begin(), c->batch_e.end(), edge_first);
+  if (!tri_first) return NUC34_OK;
+  if (!c->has_tris) return fail(c, NUC34_E_STATE, "triangles not computed");
+  return guard(c, [&]() -> int {
+    for (uint64_t g = 0; g < k; ++g)
+      if (c->batch_e[g] > c->m) throw std::logic_error("batch edge boundary out of range");
+    DevBuf idx = c->alloc.get(k * 8, "batch boundaries");
+    DevBuf out = c->alloc.get(k * 8, "batch triangle offsets");
+    NUC34_HIP(hipMemcpyAsync(idx.as<void>(), c->batch_e.data(), k * 8, hipMemcpyHostToDevice, c->s));
+    launch_gather_u64(c->offA.as<uint64_t>(), idx.as<uint64_t>(), out.as<uint64_t>(), k, c->s);
+    NUC34_HIP(hipMemcpyAsync(tri_first, out.as<void>(), k * 8, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    return NUC34_OK;
+  });
+}
+
 int nuc34_graph_dims(const nuc34_ctx* c, uint32_t* n, uint64_t* m) {
   if (!c || !c->has_graph) return NUC34_E_STATE;
   if (n) *n = c->n;
@@ -941,6 +960,11 @@ int nuc34_device_info(int device, char* buf, size_t len) {
   s += "pci_bus_id=" + std::to_string(p.pciBusID) + "\n";
   s += "hip_runtime=" + std::to_string(rt) + "\n";
   s += "hip_driver=" + std::to_string(drv) + "\n";
+  int cur = 0;
+  size_t fr = 0, tot = 0;
+  if (hipGetDevice(&cur) == hipSuccess && hipSetDevice(device) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess)
+    s += "free_memory_bytes=" + std::to_string(fr) + "\n";
+  (void)hipSetDevice(cur);
   if (!buf || !len) return NUC34_E_INVALID;
   std::strncpy(buf, s.c_str(), len - 1);
   buf[len - 1] = 0;

@@ -87,7 +87,20 @@ __global__ void k_fill(T* __restrict__ a, uint64_t n, T v) {
     a[i] = v;
 }
 
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx,
+                             uint64_t* __restrict__ out, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    out[i] = src[idx[i]];
+}
+
 }  // namespace
+
+void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, src, idx, out, n);
+  NUC34_LAUNCH_CHECK();
+}
 
 void launch_orient_count(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, uint32_t* outdeg, hipStream_t s) {
   if (!n) return;

@@ -26,6 +26,8 @@ void launch_inc_fill(const uint32_t* triA, const uint64_t* triBC, uint64_t T, co
                      const uint32_t* src, const uint64_t* incO, uint32_t* cur, uint64_t* inc, hipStream_t s);
 void launch_inc_split(const uint64_t* inc, uint64_t n, uint32_t* third, uint32_t* tri, hipStream_t s);
 
+// out[i] = src[idx[i]] for i < n (idx in bounds of src, checked by the caller).
+void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s);
 void launch_iota(uint32_t* a, uint64_t n, hipStream_t s);
 void launch_fill_i32(int32_t* a, uint64_t n, int32_t v, hipStream_t s);
 void launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t s);

@@ -29,6 +29,29 @@ class Nuc34Error(RuntimeError):
         self.code = code
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("large_fraction", C.c_double),
+                ("large_edges", C.c_uint64), ("max_batch_edges", C.c_uint64), ("max_batch_graphs", C.c_uint32),
+                ("variant", C.c_int), ("batch_window_ms", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("variant", C.c_int32), ("n", C.c_uint64), ("m", C.c_uint64),
+                ("triangles", C.c_uint64), ("four_cliques", C.c_uint64), ("k_max", C.c_int32),
+                ("batch_graphs", C.c_uint32), ("batch_id", C.c_uint64), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double), ("error", C.c_char * 96)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["error"] = self.error.decode()
+        return d
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("splits", C.c_uint64), ("busy_s", C.c_double * 2)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("ms", C.c_double), ("launches", C.c_uint64)]
 
@@ -84,7 +107,15 @@ def _load() -> C.CDLL:
         "nuc34_reset": (C.c_int, [vp]),
         "nuc34_load_batch": (C.c_int, [vp, C.c_uint32, u32p, u64p, u32p, u32p, C.c_int]),
         "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
+        "nuc34_batch_slices": (C.c_int, [vp, u64p, u64p]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
+        "nuc34_serve_create": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(vp)]),
+        "nuc34_serve_destroy": (None, [vp]),
+        "nuc34_serve_now": (C.c_double, [vp]),
+        "nuc34_serve_submit": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_uint64, u64p]),
+        "nuc34_serve_wait": (C.c_int, [vp, C.c_uint64, C.c_int64, C.POINTER(JobInfo)]),
+        "nuc34_serve_take": (C.c_int, [vp, C.c_uint64, u32p, i32p]),
+        "nuc34_serve_get_stats": (C.c_int, [vp, C.POINTER(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -251,6 +282,14 @@ class Context:
             self._check(_lib.nuc34_batch_graph(self._h, g, C.byref(vf), C.byref(ef), None, None))
         return {"vertex_first": vf.value, "edge_first": ef.value, "tri_first": tf.value, "tri_count": tc.value}
 
+    def batch_slices(self):
+        """(edge_first, tri_first) of every graph of the loaded batch, count + 1 entries each."""
+        ef = np.empty(self._batch + 1, dtype=np.uint64)
+        tf = np.empty(self._batch + 1, dtype=np.uint64)
+        p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))
+        self._check(_lib.nuc34_batch_slices(self._h, p(ef), p(tf)))
+        return ef, tf
+
     def dims(self):
         n, m = C.c_uint32(), C.c_uint64()
         self._check(_lib.nuc34_graph_dims(self._h, C.byref(n), C.byref(m)))
@@ -322,3 +361,74 @@ def decompose_graph(g: HostGraph, variant="auto", parts: int = 1, device: int = 
         ctx.load_graph(g)
         st = ctx.decompose(variant, parts)
         return ctx.copy_triangles(), ctx.copy_k(), st
+
+
+class Server:
+    """Decomposition service (nuc34_serve_*): submit undirected graphs from any
+    thread, wait for and take their results. Small jobs are batched as disjoint
+    unions on one lane, large ones run alone on a second lane; both lanes share
+    the device through their own streams."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, large_fraction: float = 0.0, large_edges: int = 0,
+                 max_batch_edges: int = 0, max_batch_graphs: int = 0, variant="auto", batch_window_ms: float = 0.0):
+        cfg = ServeConfig(device, hbm_budget, large_fraction, large_edges, max_batch_edges, max_batch_graphs,
+                          VARIANTS[variant] if isinstance(variant, str) else int(variant), batch_window_ms)
+        self._h = C.c_void_p()
+        rc = _lib.nuc34_serve_create(C.byref(cfg), C.byref(self._h))
+        if rc != OK:
+            raise Nuc34Error(rc, "nuc34_serve_create failed")
+        self._T = {}
+
+    def close(self):
+        if self._h:
+            _lib.nuc34_serve_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self) -> float:
+        return _lib.nuc34_serve_now(self._h)
+
+    def submit(self, g) -> int:
+        O, E = (g.O, g.E) if isinstance(g, HostGraph) else g
+        O = np.ascontiguousarray(O, dtype=np.uint32)
+        E = np.ascontiguousarray(E, dtype=np.uint32)
+        job = C.c_uint64()
+        rc = _lib.nuc34_serve_submit(self._h, _u32(O), _u32(E if len(E) else O), len(O) - 1, len(E), C.byref(job))
+        if rc != OK:
+            raise Nuc34Error(rc, "submit rejected")
+        return job.value
+
+    def wait(self, job: int, timeout_ms: int = -1) -> dict | None:
+        info = JobInfo()
+        rc = _lib.nuc34_serve_wait(self._h, job, timeout_ms, C.byref(info))
+        if rc == E_STATE:
+            return None
+        if rc != OK:
+            raise Nuc34Error(rc, "wait failed")
+        return info.as_dict()
+
+    def take(self, job: int, triangles: int):
+        """Triangles (T,3) with the graph's own oriented edge ids, and k (T,); forgets the job."""
+        abc = np.empty((max(triangles, 1), 3), dtype=np.uint32)
+        k = np.empty(max(triangles, 1), dtype=np.int32)
+        rc = _lib.nuc34_serve_take(self._h, job, _u32(abc), _i32(k))
+        if rc != OK:
+            raise Nuc34Error(rc, "take failed (job unknown or not done)")
+        return abc[:triangles], k[:triangles]
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _lib.nuc34_serve_get_stats(self._h, C.byref(st))
+        return {"submitted": st.submitted, "completed": st.completed, "failed": st.failed, "batches": st.batches,
+                "splits": st.splits, "busy_s": [st.busy_s[0], st.busy_s[1]]}

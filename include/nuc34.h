@@ -135,6 +135,9 @@ int nuc34_load_batch(nuc34_ctx* ctx, uint32_t count, const uint32_t* n, const ui
  * indices of graph g minus edge_first are the graph's own. */
 int nuc34_batch_graph(nuc34_ctx* ctx, uint32_t g, uint64_t* vertex_first, uint64_t* edge_first, uint64_t* tri_first,
                       uint64_t* tri_count);
+/* All graphs' boundaries at once: edge_first and tri_first receive count + 1
+ * entries each (the last one is the union's total); either may be NULL. */
+int nuc34_batch_slices(nuc34_ctx* ctx, uint64_t* edge_first, uint64_t* tri_first);
 int nuc34_copy_oriented(nuc34_ctx* ctx, uint32_t* O, uint32_t* E);
 
 /* Triangles in the reference's canonical (a,b) order; `parts` >= 1 splits the
@@ -163,6 +166,70 @@ int nuc34_reset(nuc34_ctx* ctx);
 
 /* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
 int nuc34_device_info(int device, char* buf, size_t len);
+
+/* ---- serving loop -------------------------------------------------------- */
+/*
+ * No counterpart in the reference (one graph per process): a decomposition
+ * service for a stream of independent graphs. Any thread submits jobs; two
+ * worker lanes, each with its own context and stream on the device, take them
+ * in arrival order. The small-job lane packs every queued job that fits its
+ * caps into one disjoint-union batch (one pass of every kernel, shared peel
+ * rounds) as soon as it is free; the large-job lane runs graphs at or above
+ * `large_edges` one at a time, so they never hold small jobs back. Admission
+ * is by HBM footprint: each batch's variant is chosen after the support count
+ * (direct when its K4 lists fit the lane's budget, else on-demand), and a
+ * batch that still runs out of its budget is split in halves and retried.
+ */
+typedef struct nuc34_server nuc34_server;
+
+typedef struct nuc34_serve_config {
+  int device;
+  uint64_t hbm_budget;       /* bytes for both lanes (0 = 90% of the free device memory) */
+  double large_fraction;     /* share of the budget for the large-job lane (0 = 0.75) */
+  uint64_t large_edges;      /* undirected edge entries (2 per edge) at which a job is large (0 = 4 Mi) */
+  uint64_t max_batch_edges;  /* edge entries per small-job batch (0 = 256 Mi) */
+  uint32_t max_batch_graphs; /* graphs per small-job batch (0 = 4096) */
+  int variant;               /* NUC34_AUTO (default), NUC34_DIRECT or NUC34_ONDEMAND */
+  double batch_window_ms;    /* small-job lane: wait this long after its oldest queued job
+                                arrived before starting a batch (0 = start at once) */
+} nuc34_serve_config;
+
+typedef struct nuc34_job_info {
+  int32_t status;        /* NUC34_OK, or the failure code of the job's last attempt */
+  int32_t variant;       /* what ran */
+  uint64_t n;            /* vertices */
+  uint64_t m;            /* oriented edges */
+  uint64_t triangles;
+  uint64_t four_cliques;
+  int32_t k_max;
+  uint32_t batch_graphs; /* jobs decomposed together with this one (itself included) */
+  uint64_t batch_id;
+  double t_submit;       /* seconds on the server clock (nuc34_serve_now) */
+  double t_start;        /* its batch started */
+  double t_done;         /* its results are on the host */
+  char error[96];
+} nuc34_job_info;
+
+typedef struct nuc34_serve_stats {
+  uint64_t submitted, completed, failed, batches, splits;
+  double busy_s[2];      /* device-side wall time per lane (small, large) */
+} nuc34_serve_stats;
+
+int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out);
+/* Drains the queues, then stops the lanes and frees everything. */
+void nuc34_serve_destroy(nuc34_server* srv);
+double nuc34_serve_now(const nuc34_server* srv);
+/* Queues an undirected CSR (as nuc34_load_csr with oriented = 0); the server
+ * copies it before returning. */
+int nuc34_serve_submit(nuc34_server* srv, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m,
+                       uint64_t* job);
+/* Waits up to timeout_ms (negative: no limit) for the job; NUC34_E_STATE when
+ * it is not done by then or unknown. */
+int nuc34_serve_wait(nuc34_server* srv, uint64_t job, int64_t timeout_ms, nuc34_job_info* info);
+/* Copies a finished job's triangles (3T: a, b, c as the graph's own oriented
+ * edge ids, (a,b) order) and k (T); either may be NULL. Then forgets the job. */
+int nuc34_serve_take(nuc34_server* srv, uint64_t job, uint32_t* abc, int32_t* k);
+int nuc34_serve_get_stats(nuc34_server* srv, nuc34_serve_stats* st);
 
 #ifdef __cplusplus
 }

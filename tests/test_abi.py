@@ -36,3 +36,31 @@ def test_python_mirror_binds_all():
 def test_no_cpu_fallback_in_mirror():
     src = open(os.path.join(ROOT, "gpu-nucleus34_amd", "nuc34", "__init__.py")).read()
     assert "oracle" not in src.replace("oracle/", "")
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the ABI structs have the C layout (sizes and offsets)."""
+    import ctypes
+    import subprocess
+    import nuc34
+    fields = {"nuc34_serve_config": (nuc34.ServeConfig, ["hbm_budget", "large_fraction", "variant", "batch_window_ms"]),
+              "nuc34_job_info": (nuc34.JobInfo, ["n", "t_submit", "t_done", "error"]),
+              "nuc34_serve_stats": (nuc34.ServeStats, ["splits", "busy_s"]),
+              "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels"])}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "nuc34.h"', "int main(void) {"]
+    for st, (_, fs) in fields.items():
+        src.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            src.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-I", inc, str(c), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                                check=True).stdout.split("\n") if line)
+    for st, (cls, fs) in fields.items():
+        assert int(out[st]) == ctypes.sizeof(cls), st
+        for f in fs:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

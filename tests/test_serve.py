@@ -1,0 +1,96 @@
+"""Serving loop (nuc34_serve_*): jobs submitted from several threads come back
+bit-exact against their own single-graph decompositions (and the smallest
+against the oracle), whichever lane and batch ran them; a batch that exceeds
+its lane's budget is split and still returns exact results."""
+import threading
+
+import numpy as np
+import pytest
+
+import nuc34
+import oracle34
+
+pytestmark = pytest.mark.gpu
+
+
+def single(g, variant="auto"):
+    with nuc34.Context(0) as ctx:
+        ctx.load_graph(g)
+        st = ctx.decompose(variant)
+        return ctx.copy_triangles(), ctx.copy_k(), st
+
+
+def check_jobs(srv, jobs, gs, variant="auto"):
+    infos = [srv.wait(j, 120000) for j in jobs]
+    assert all(i is not None for i in infos), "job did not finish"
+    for j, info, g in zip(jobs, infos, gs):
+        assert info["status"] == nuc34.OK, info["error"]
+        tris, k = srv.take(j, info["triangles"])
+        t1, k1, st1 = single(g, variant)
+        assert info["triangles"] == st1["triangles"] and info["four_cliques"] == st1["four_cliques"]
+        assert info["t_submit"] <= info["t_start"] <= info["t_done"]
+        assert np.array_equal(tris, t1) and np.array_equal(k, k1)
+        assert info["k_max"] == (int(k1.max()) if len(k1) else 0)
+    return infos
+
+
+def test_serve_two_lanes_threads():
+    print("[serve] two lanes, 3 submitting threads, 30 jobs", flush=True)
+    gs = [nuc34.gen_batch_graph(4, i) for i in range(30)]
+    large = 150_000  # edge entries: the bigger graphs of the mix go to the large-job lane
+    jobs = [None] * len(gs)
+    with nuc34.Server(hbm_budget=16 << 30, large_edges=large) as srv:
+        def submit(lo):
+            for i in range(lo, len(gs), 3):
+                jobs[i] = srv.submit(gs[i])
+        th = [threading.Thread(target=submit, args=(r,)) for r in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        infos = check_jobs(srv, jobs, gs)
+        st = srv.stats()
+    assert st["completed"] == len(gs) and st["failed"] == 0
+    n_large = sum(g.m >= large for g in gs)
+    assert 0 < n_large < len(gs)
+    for info, g in zip(infos, gs):
+        if g.m >= large:
+            assert info["batch_graphs"] == 1
+    order = sorted(range(len(gs)), key=lambda i: gs[i].m)
+    for i in order[:2]:
+        ref = oracle34.OracleGraph.from_csr(gs[i].O, gs[i].E, orient=True).run("host", 4)
+        t1, k1, _ = single(gs[i])
+        assert np.array_equal(k1, ref["k"]) and np.array_equal(t1, ref["tris"])
+
+
+def test_serve_batches_and_splits_on_budget():
+    print("[serve] forced direct, small-lane budget below the union's footprint", flush=True)
+    gs = [nuc34.gen_batch_graph(4, i) for i in (0, 3, 6, 9, 12, 15, 18, 21)]
+    peaks = [single(g, "direct")[2]["peak_bytes"] for g in gs]
+    with nuc34.Context(0) as ctx:
+        ctx.load_batch(gs)
+        union_peak = ctx.decompose("direct")["peak_bytes"]
+    if union_peak < 1.2 * max(peaks):
+        pytest.skip(f"union footprint {union_peak} too close to the largest single job's {max(peaks)}")
+    budget = (max(peaks) + union_peak) // 2  # every job fits alone, the whole batch does not
+    # a 2 s batching window gathers all eight jobs into the first batch
+    with nuc34.Server(hbm_budget=2 * budget, large_fraction=0.5, variant="direct", batch_window_ms=2000) as srv:
+        jobs = [srv.submit(g) for g in gs]
+        infos = check_jobs(srv, jobs, gs, "direct")
+        st = srv.stats()
+    print(f"[serve] batches={st['batches']} splits={st['splits']}", flush=True)
+    assert st["completed"] == len(gs) and st["failed"] == 0
+    assert st["splits"] >= 1 and max(i["batch_graphs"] for i in infos) < len(gs)
+
+
+def test_serve_empty_and_triangle_free():
+    with nuc34.Server(hbm_budget=1 << 30) as srv:
+        empty = (np.zeros(1, np.uint32), np.zeros(0, np.uint32))
+        path = (np.array([0, 1, 3, 4], np.uint32), np.array([1, 0, 2, 1], np.uint32))
+        j0, j1 = srv.submit(empty), srv.submit(path)
+        for j in (j0, j1):
+            info = srv.wait(j, 60000)
+            assert info["status"] == nuc34.OK and info["triangles"] == 0 and info["k_max"] == 0
+            tris, k = srv.take(j, 0)
+            assert len(tris) == 0 and len(k) == 0
+        assert srv.wait(j0, 0) is None  # taken jobs are forgotten
