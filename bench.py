@@ -73,12 +73,14 @@ def algorithmic_bytes(kernel: str, g: dict, st: dict) -> float:
         return 24 * m + 4 * g["P"] + 4 * m + T * (12 + 8)
     if kernel == "support_bits":
         return T * (12 + 12) + 2 * 4 * m  # triangle record in, three counts out, rows read once
-    if kernel == "k4_owned":
-        return T * 12 + Q * 8
-    if kernel == "k4_place":
-        return Q * (8 + 4 * 12)  # owned-K4 record in, four 12-byte list entries out
+    if kernel == "k4_lists":
+        # triangle record and list offset in, four 12-byte entries per K4 out
+        return T * (12 + 8) + Q * 4 * 12
     if kernel == "peel_rounds":
-        return 4 * Q * 12 + 3 * Q * 8 + T * 20  # every list once, debits, frontier + offsets
+        # every K4 entry once (read from the lists, or produced by the on-demand
+        # enumeration), one 8-byte atomic read-modify-write per debit (at most
+        # 3 per K4), support/state/k/frontier traffic per triangle
+        return 4 * Q * 12 + 3 * Q * 8 + T * 20
     return 0.0
 
 
@@ -88,9 +90,8 @@ GROUP_KERNELS = {
     "tri_enum_bits": ("k_tri_enum<false>",),
     "tri_enum_emit": ("k_tri_enum<true>",),
     "support_bits": ("k_support_bits",),
-    "k4_owned": ("k_k4_owned",),
-    "k4_place": ("k_k4_place",),
-    "peel_rounds": ("k_peel_level", "k_peel_collect", "k_peel_debit", "k_peel_finish"),
+    "k4_lists": ("k_k4_lists",),
+    "peel_rounds": ("k_peel_collect", "k_peel_debit", "k_peel_finish"),
 }
 
 

@@ -1,7 +1,7 @@
 // C ABI of the (3,4)-nucleus path: a device context that owns every device
 // buffer (no global device symbols, so contexts are reentrant), and the stage
-// orchestration: orientation -> triangles -> incidence -> support ->
-// [K4 list + triangle->K4 incidence] -> peel.
+// orchestration: orientation -> triangles + bit-packed incidence -> support ->
+// [per-triangle K4 lists] -> peel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,13 +44,11 @@ struct nuc34_ctx {
   DevBuf IO, IE, IEe, inpos;        // in-CSR of the oriented graph
   DevBuf rowA, rowC, Aout, Ain, Cm;  // bit rows
   DevBuf offA, triA, triBC;
+  DevBuf cIdx;                       // triangle id per c-slot
   // support
   bool has_sup = false;
   uint64_t Q = 0;
-  DevBuf sup, supUp, own, small;  // small: counters
-  // third-vertex-sorted incidence (on-demand re-enumeration)
-  bool has_inc = false;
-  DevBuf incO, third, tri, bigq;
+  DevBuf sup, own, small;  // small: counters
   // direct: per-triangle K4 partner lists
   bool has_k4 = false;
   DevBuf koff, lists;
@@ -106,9 +104,9 @@ double timed(nuc34_ctx* c, F&& f) {
   return ms;
 }
 
-enum KernelId { K_TRI_BITS, K_TRI_EMIT, K_SUPPORT, K_K4_OWNED, K_K4_PLACE, K_PEEL, K_COUNT };
-const char* const kKernelNames[K_COUNT] = {"tri_enum_bits", "tri_enum_emit", "support_bits",
-                                          "k4_owned",      "k4_place",      "peel_rounds"};
+enum KernelId { K_TRI_BITS, K_TRI_EMIT, K_SUPPORT, K_K4_LISTS, K_PEEL, K_COUNT };
+const char* const kKernelNames[K_COUNT] = {"tri_enum_bits", "tri_enum_emit", "support_bits", "k4_lists",
+                                          "peel_rounds"};
 
 // Records events around `f` on the context stream; read back by ktime_collect.
 template <typename F>
@@ -174,24 +172,23 @@ void zero(nuc34_ctx* c, DevBuf& b) {
 }
 
 void reset_stages(nuc34_ctx* c, int from) {
-  // from: 0 graph, 1 triangles, 2 support, 3 k4 lists / incidence, 4 k
+  // from: 0 graph, 1 triangles, 2 support, 3 k4 lists, 4 k
   if (from <= 4) {
     c->kv.release();
     c->has_k = false;
   }
   if (from <= 3) {
-    for (DevBuf* b : {&c->koff, &c->lists, &c->incO, &c->third, &c->tri, &c->bigq}) b->release();
+    for (DevBuf* b : {&c->koff, &c->lists}) b->release();
     c->has_k4 = false;
-    c->has_inc = false;
   }
   if (from <= 2) {
-    for (DevBuf* b : {&c->sup, &c->supUp, &c->own, &c->small}) b->release();
+    for (DevBuf* b : {&c->sup, &c->own, &c->small}) b->release();
     c->has_sup = false;
     c->Q = 0;
   }
   if (from <= 1) {
     for (DevBuf* b : {&c->IO, &c->IE, &c->IEe, &c->inpos, &c->rowA, &c->rowC, &c->Aout, &c->Ain, &c->Cm, &c->offA,
-                      &c->triA, &c->triBC})
+                      &c->triA, &c->triBC, &c->cIdx})
       b->release();
     c->has_tris = false;
     c->T = 0;
@@ -227,13 +224,14 @@ std::vector<std::pair<uint64_t, uint64_t>> edge_parts(nuc34_ctx* c, uint32_t par
 }
 
 BitRows bitrows(nuc34_ctx* c) {
-  return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<uint32_t>(),
-                 c->Cm.as<uint32_t>()};
+  return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<RankWord>(),
+                 c->Cm.as<RankWord>()};
 }
 
-TriView triview(nuc34_ctx* c) {
-  return TriView{c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->T, c->incO.as<uint64_t>(),
-                 c->third.as<uint32_t>(), c->tri.as<uint32_t>()};
+K4Src k4src(nuc34_ctx* c) {
+  return K4Src{c->O.as<uint32_t>(),    c->E.as<uint32_t>(),     c->src.as<uint32_t>(), c->inpos.as<uint32_t>(),
+               bitrows(c),             c->triA.as<uint32_t>(),  c->triBC.as<uint64_t>(), c->cIdx.as<uint32_t>(),
+               c->T};
 }
 
 void run_triangles(nuc34_ctx* c, uint32_t parts) {
@@ -263,8 +261,8 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     const uint64_t WA = read_scalar<uint64_t>(c, c->rowA.as<uint64_t>() + m);
     const uint64_t WC = read_scalar<uint64_t>(c, c->rowC.as<uint64_t>() + m);
     c->Aout = c->alloc.get(WA * 4 + 4, "b-role rows");
-    c->Ain = c->alloc.get(WA * 4 + 4, "a-role rows");
-    c->Cm = c->alloc.get(WC * 4 + 4, "c-role rows");
+    c->Ain = c->alloc.get(WA * 8 + 8, "a-role rows");
+    c->Cm = c->alloc.get(WC * 8 + 8, "c-role rows");
     zero(c, c->Aout);
     zero(c, c->Ain);
     zero(c, c->Cm);
@@ -276,21 +274,31 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     });
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
+      // a-role counts give each edge's first triangle id, c-role counts its
+      // first c-slot; both become the running ranks of the row words
       DevBuf cntA = c->alloc.get(m * 4 + 4, "a-role counts");
-      launch_row_counts(rows, m, cntA.as<uint32_t>(), nullptr, nullptr, c->s);
+      DevBuf cntC = c->alloc.get(m * 4 + 4, "c-role counts");
+      DevBuf cOff = c->alloc.get((m + 1) * 8, "c-slot offsets");
+      launch_row_counts(rows, m, cntA.as<uint32_t>(), cntC.as<uint32_t>(), c->s);
       DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
       exclusive_scan<uint32_t, uint64_t>(cntA.as<uint32_t>(), c->offA.as<uint64_t>(), m, scratch.as<uint64_t>(),
                                          c->s);
+      exclusive_scan<uint32_t, uint64_t>(cntC.as<uint32_t>(), cOff.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
       c->T = read_scalar<uint64_t>(c, c->offA.as<uint64_t>() + m);
+      if (c->T >= 0xFFFFFFFFull)
+        throw Overflow("triangle count " + std::to_string(c->T) + " exceeds the 32-bit triangle id range");
+      if (read_scalar<uint64_t>(c, cOff.as<uint64_t>() + m) != c->T)
+        throw std::logic_error("c-role bits disagree with the triangle count");
+      launch_row_ranks(rows, m, c->offA.as<uint64_t>(), cOff.as<uint64_t>(), c->s);
     }
-    if (c->T >= 0xFFFFFFFFull)
-      throw Overflow("triangle count " + std::to_string(c->T) + " exceeds the 32-bit triangle id range");
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
+    c->cIdx = c->alloc.get(c->T * 4 + 4, "triangles by c-slot");
     ktime(c, K_TRI_EMIT, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
-                        c->offA.as<uint64_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->s);
+                        c->inpos.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
+                        c->cIdx.as<uint32_t>(), c->s);
     });
   });
   c->has_tris = true;
@@ -302,13 +310,11 @@ void run_support(nuc34_ctx* c) {
   const uint64_t T = c->T;
   c->st.ms_support = timed(c, [&] {
     c->sup = c->alloc.get(T * 4 + 4, "support");
-    c->supUp = c->alloc.get(T * 4 + 4, "support above");
     c->own = c->alloc.get(T * 4 + 4, "owned K4 counts");
     c->small = c->alloc.get(64, "counters");
     zero(c, c->small);
     ktime(c, K_SUPPORT, 1, [&] {
-      launch_support_bits(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), c->sup.as<int32_t>(),
-                          c->supUp.as<uint32_t>(), c->own.as<uint32_t>(),
+      launch_support_bits(k4src(c), c->sup.as<int32_t>(), c->own.as<uint32_t>(),
                           reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->s);
     });
     const uint64_t total = read_scalar<uint64_t>(c, c->small.as<char>() + 8);
@@ -317,42 +323,6 @@ void run_support(nuc34_ctx* c) {
   });
   c->has_sup = true;
   c->st.four_cliques = c->Q;
-}
-
-// Third-vertex-sorted edge->triangle lists, the on-demand peel's re-enumeration
-// structure (build_incidence_entries + sort_by_key, cuNucleus34_ondemand.cu:502-555).
-void run_incidence(nuc34_ctx* c) {
-  const uint64_t m = c->m, T = c->T;
-  c->st.ms_incidence = timed(c, [&] {
-    DevBuf len = c->alloc.get(m * 4 + 4, "incidence lengths");
-    {
-      DevBuf cA = c->alloc.get(m * 4 + 4, "role counts");
-      DevBuf cB = c->alloc.get(m * 4 + 4, "role counts");
-      DevBuf cC = c->alloc.get(m * 4 + 4, "role counts");
-      launch_row_counts(bitrows(c), m, cA.as<uint32_t>(), cB.as<uint32_t>(), cC.as<uint32_t>(), c->s);
-      launch_inc_len(cA.as<uint32_t>(), cB.as<uint32_t>(), cC.as<uint32_t>(), m, len.as<uint32_t>(), c->s);
-    }
-    c->incO = c->alloc.get((m + 1) * 8, "incidence offsets");
-    {
-      DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(len.as<uint32_t>(), c->incO.as<uint64_t>(), m, scratch.as<uint64_t>(),
-                                         c->s);
-    }
-    len.release();
-    DevBuf inc = c->alloc.get(3 * T * 8 + 8, "incidence entries");
-    {
-      DevBuf cur = c->alloc.get(m * 4 + 4, "incidence cursors");
-      zero(c, cur);
-      launch_inc_fill(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, c->E.as<uint32_t>(),
-                      c->src.as<uint32_t>(), c->incO.as<uint64_t>(), cur.as<uint32_t>(), inc.as<uint64_t>(), c->s);
-    }
-    segmented_sort_u64(inc.as<uint64_t>(), c->incO.as<uint64_t>(), m, c->alloc, c->s);
-    c->third = c->alloc.get(3 * T * 4 + 4, "incidence third vertices");
-    c->tri = c->alloc.get(3 * T * 4 + 4, "incidence triangles");
-    launch_inc_split(inc.as<uint64_t>(), 3 * T, c->third.as<uint32_t>(), c->tri.as<uint32_t>(), c->s);
-    c->bigq = c->alloc.get(T * 4 + 4, "wave queue");
-  });
-  c->has_inc = true;
 }
 
 void run_k4_lists(nuc34_ctx* c) {
@@ -364,26 +334,10 @@ void run_k4_lists(nuc34_ctx* c) {
       exclusive_scan<int32_t, uint64_t>(c->sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
                                         c->s);
     }
+    if (read_scalar<uint64_t>(c, c->koff.as<uint64_t>() + T) != 4 * Q)
+      throw std::logic_error("K4 list sizes disagree with the support sum");
     c->lists = c->alloc.get(4 * Q * 12 + 12, "K4 partner lists");
-    DevBuf k4off = c->alloc.get((T + 1) * 8, "owned K4 offsets");
-    {
-      DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
-                                         c->s);
-    }
-    if (read_scalar<uint64_t>(c, k4off.as<uint64_t>() + T) != Q)
-      throw std::logic_error("owned K4 count disagrees with the support sum");
-    DevBuf k4 = c->alloc.get(Q * 8 + 8, "owned K4s");
-    ktime(c, K_K4_OWNED, 1, [&] {
-      launch_k4_owned(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), k4off.as<uint64_t>(),
-                      k4.as<uint64_t>(), c->s);
-    });
-    ktime(c, K_K4_PLACE, 1, [&] {
-      launch_k4_place(k4.as<uint64_t>(), Q, c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->O.as<uint32_t>(),
-                      c->E.as<uint32_t>(), c->src.as<uint32_t>(), c->inpos.as<uint32_t>(), bitrows(c),
-                      c->offA.as<uint64_t>(), c->koff.as<uint64_t>(), c->supUp.as<uint32_t>(),
-                      c->lists.as<uint32_t>(), c->s);
-    });
+    ktime(c, K_K4_LISTS, 1, [&] { launch_k4_lists(k4src(c), c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s); });
   });
   c->has_k4 = true;
 }
@@ -411,9 +365,8 @@ void run_peel(nuc34_ctx* c, int variant) {
   DevBuf b1 = c->alloc.get(T * 4, "frontier");
   DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
-              b0.as<uint32_t>(), b1.as<uint32_t>(),  variant == NUC34_DIRECT ? nullptr : c->bigq.as<uint32_t>(),
-              ctlb.as<PeelCtl>(), T};
-  const TriView v = triview(c);
+              b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
+  const K4Src v = k4src(c);
   static_assert(kRoundsPerLaunch % 2 == 0, "round parity must equal slot parity");
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {
@@ -862,22 +815,17 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
       v = (direct_bytes(c) + c->T * 28 <= room) ? NUC34_DIRECT : NUC34_ONDEMAND;
     }
     reset_stages(c, 4);
-    if (c->Q) {
-      if (v == NUC34_DIRECT && !c->has_k4) {
-        c->st.ms_incidence = 0;
-        run_k4_lists(c);
-      }
-      if (v == NUC34_ONDEMAND && !c->has_inc) {
-        c->st.ms_k4_list = 0;
-        run_incidence(c);
-      }
-    }
+    // the on-demand variant needs nothing beyond the bit rows: each dying
+    // triangle's K4s are enumerated from them in the peel
+    c->st.ms_incidence = 0;
+    if (v != NUC34_DIRECT) c->st.ms_k4_list = 0;
+    if (c->Q && v == NUC34_DIRECT && !c->has_k4) run_k4_lists(c);
     run_peel(c, v);
     c->st.variant = v;
     c->st.peak_bytes = c->alloc.peak;
     ktime_collect(c);
-    c->st.ms_total = c->st.ms_triangles + c->st.ms_support +
-                     (v == NUC34_DIRECT ? c->st.ms_k4_list : c->st.ms_incidence) + c->st.ms_peel;
+    c->st.ms_total = c->st.ms_triangles + c->st.ms_support + (v == NUC34_DIRECT ? c->st.ms_k4_list : 0.0) +
+                     c->st.ms_peel;
     if (stats) *stats = c->st;
     return NUC34_OK;
   });
@@ -907,12 +855,9 @@ int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_
                                          c->s);
     }
     DevBuf k4 = c->alloc.get(c->Q * 8 + 8, "owned K4s");
-    launch_k4_owned(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), T, bitrows(c), k4off.as<uint64_t>(),
-                    k4.as<uint64_t>(), c->s);
+    launch_k4_owned(k4src(c), k4off.as<uint64_t>(), k4.as<uint64_t>(), c->s);
     DevBuf out = c->alloc.get(count * 16, "K4 copy");
-    launch_k4_quads(k4.as<uint64_t>(), first, count, c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                    c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), bitrows(c),
-                    c->offA.as<uint64_t>(), out.as<uint32_t>(), c->s);
+    launch_k4_quads(k4src(c), k4.as<uint64_t>(), first, count, out.as<uint32_t>(), c->s);
     NUC34_HIP(hipMemcpyAsync(quds, out.as<void>(), count * 16, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;

@@ -48,19 +48,34 @@ __global__ void k_row_words(const uint32_t* __restrict__ O, const uint32_t* __re
   }
 }
 
-__global__ void k_row_counts(BitRows r, uint64_t m, uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntB,
-                             uint32_t* __restrict__ cntC) {
+__global__ void k_row_counts(BitRows r, uint64_t m, uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntC) {
   for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
        e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t a = 0, b = 0, c = 0;
-    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i) {
-      a += static_cast<uint32_t>(__popc(r.Ain[i]));
-      b += static_cast<uint32_t>(__popc(r.Aout[i]));
-    }
-    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) c += static_cast<uint32_t>(__popc(r.Cm[i]));
+    uint32_t a = 0, c = 0;
+    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i) a += static_cast<uint32_t>(__popc(r.Ain[i].bits));
+    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) c += static_cast<uint32_t>(__popc(r.Cm[i].bits));
     cntA[e] = a;
-    if (cntB) cntB[e] = b;
-    if (cntC) cntC[e] = c;
+    cntC[e] = c;
+  }
+}
+
+// Running ranks: every Ain word of e gets offA[e] plus the set bits of e's row
+// before it, every Cm word cOff[e] plus the bits before it. Ranks are triangle
+// ids and c-slots, which stay below 2^32 (checked by the caller).
+__global__ void k_row_ranks(BitRows r, uint64_t m, const uint64_t* __restrict__ offA,
+                            const uint64_t* __restrict__ cOff) {
+  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t run = static_cast<uint32_t>(offA[e]);
+    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i) {
+      r.Ain[i].rank = run;
+      run += static_cast<uint32_t>(__popc(r.Ain[i].bits));
+    }
+    run = static_cast<uint32_t>(cOff[e]);
+    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) {
+      r.Cm[i].rank = run;
+      run += static_cast<uint32_t>(__popc(r.Cm[i].bits));
+    }
   }
 }
 
@@ -107,10 +122,15 @@ void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC,
-                       hipStream_t s) {
+void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s) {
   if (!m) return;
-  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cntA, cntB, cntC);
+  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cntA, cntC);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_row_ranks(const BitRows& r, uint64_t m, const uint64_t* offA, const uint64_t* cOff, hipStream_t s) {
+  if (!m) return;
+  hipLaunchKernelGGL(k_row_ranks, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, offA, cOff);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -7,32 +7,43 @@
 // (vertices named by rank as in the reference's tri_t: a=(u,w), b=(u,v),
 // c=(v,w), u<v<w; nucleus/utils.cuh:93-102). Aout/Ain rows of e hold
 // ceil(d+(u)/32) words, Cm rows ceil(d-(u)/32) words, at per-edge word offsets.
+// (Aout|Ain) of e=(u,y) is y's adjacency inside N+(u).
 //
-// With these rows:
-//   * triangle index       = offA[a] + (bits of Ain[a] below v's position)
+// Ain and Cm words carry a running rank beside their bits: the rank of an Ain
+// word of edge e is offA[e] plus the set bits of e's row before that word, so
+// the id of the triangle at bit j of the row is rank + popc(bits below j) —
+// one 8-byte read; likewise a Cm word's rank is the c-slot offset cOff[e] plus
+// the bits before it, and cIdx[c-slot] holds the triangle id. With these:
+//   * triangle index of (u, N+(u)[j], w) = row_rank(Ain row of (u,w), j)
 //   * K4s of t=(u,v,w) above u: bits of (Aout|Ain)[a] & (Aout|Ain)[b]
 //     K4s of t below u:          bits of Cm[a] & Cm[b]
-//   * the three partner triangles of each K4 follow from the same prefix counts.
+//   * each partner triangle of each K4 is one such rank lookup away.
 #pragma once
 #include "common.hpp"
 
 namespace nuc34 {
 
+struct RankWord {
+  uint32_t bits;
+  uint32_t rank;
+};
+
 struct BitRows {
   const uint64_t* rowA;  // m + 1 word offsets of the Aout / Ain rows
   const uint64_t* rowC;  // m + 1 word offsets of the Cm rows
   uint32_t* Aout;
-  uint32_t* Ain;
-  uint32_t* Cm;
+  RankWord* Ain;
+  RankWord* Cm;
 };
 
-// Set bits of row[0..) strictly below bit position j.
-__device__ __forceinline__ uint32_t bits_below(const uint32_t* row, uint32_t j) {
-  uint32_t n = 0;
-  const uint32_t full = j >> 5;
-  for (uint32_t i = 0; i < full; ++i) n += static_cast<uint32_t>(__popc(row[i]));
-  if (j & 31) n += static_cast<uint32_t>(__popc(row[full] & ((1u << (j & 31)) - 1u)));
-  return n;
+// Rank of bit j (j < 32) of a ranked word: its rank plus the set bits below j.
+__device__ __forceinline__ uint32_t rank_in_word(RankWord w, uint32_t j) {
+  return w.rank + static_cast<uint32_t>(__popc(w.bits & ((1u << j) - 1u)));
+}
+
+// Rank of bit position j of a ranked row.
+__device__ __forceinline__ uint32_t row_rank(const RankWord* row, uint32_t j) {
+  return rank_in_word(row[j >> 5], j & 31);
 }
 
 // Set bits of (x & y) strictly below bit position j.
@@ -44,17 +55,6 @@ __device__ __forceinline__ uint32_t and_bits_below(const uint32_t* x, const uint
   return n;
 }
 
-// Set bits of (x1|x2) & (y1|y2) strictly below bit position j.
-__device__ __forceinline__ uint32_t sym_bits_below(const uint32_t* x1, const uint32_t* x2, const uint32_t* y1,
-                                                   const uint32_t* y2, uint32_t j) {
-  uint32_t n = 0;
-  const uint32_t full = j >> 5;
-  for (uint32_t i = 0; i < full; ++i) n += static_cast<uint32_t>(__popc((x1[i] | x2[i]) & (y1[i] | y2[i])));
-  if (j & 31)
-    n += static_cast<uint32_t>(__popc((x1[full] | x2[full]) & (y1[full] | y2[full]) & ((1u << (j & 31)) - 1u)));
-  return n;
-}
-
 // In-CSR (transpose of the oriented graph) with lists sorted by source id,
 // plus inpos[e] = position of edge e in its target's in-list.
 void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint32_t n, uint64_t m, uint32_t* IO,
@@ -62,7 +62,9 @@ void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uin
 // Per-edge row lengths in words: wa[e] = ceil(d+(src e)/32), wc[e] = ceil(d-(src e)/32).
 void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src, uint64_t m, uint32_t* wa,
                       uint32_t* wc, hipStream_t s);
-// Role counts from the rows: cntA = |Ain|, cntB = |Aout|, cntC = |Cm| per edge.
-void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntB, uint32_t* cntC, hipStream_t s);
+// Role counts from the rows: cntA = |Ain|, cntC = |Cm| per edge.
+void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s);
+// Running ranks of the Ain words (from offA) and Cm words (from cOff).
+void launch_row_ranks(const BitRows& r, uint64_t m, const uint64_t* offA, const uint64_t* cOff, hipStream_t s);
 
 }  // namespace nuc34

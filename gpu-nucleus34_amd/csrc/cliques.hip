@@ -1,23 +1,37 @@
-// (c) four-clique support / enumeration, (d) on-demand re-enumeration and (e)
-// the level-synchronous peel.
+// (c) four-clique support, (d) the per-triangle K4 lists and (e) the
+// level-synchronous peel, over one K4 enumeration from the ranked bit rows.
 //
-// A four-clique on triangle t (vertices u<v<w, edges a=(u,w), b=(u,v), c=(v,w))
-// is a vertex x that appears as third vertex in the incidence lists of all of
-// b, a and c; the matching entries carry the three partner triangles
-// (for_each_clique_of, nucleus/cuNucleus34_ondemand.cu:136-170). Lists whose
-// combined length is at most kSmallL are intersected by one lane (3-way
-// zipper); longer ones by a whole wave, 64 entries of each list at a time: the
-// lanes hold the chunk of b's list, the chunks of a's and c's lists sit in LDS,
-// every entry not above the smallest chunk maximum is resolved by binary search
-// and each list advances past the resolved entries.
+// K4 enumeration. For triangle t = (u, v, w) with a = (u,w), b = (u,v),
+// c = (v,w) (ranks u < v < w), the fourth vertices x are
+//   up:   bits jx of (Aout|Ain)[a] & (Aout|Ain)[b]  (x = N+(u)[jx], x adjacent
+//         to v and w inside N+(u));
+//   down: bits i of Cm[a] & Cm[b]                   (x = N-(u)[i]).
+// The partner triangles of each K4 are rank lookups in the rows:
+//   up, ex = (u,x):
+//     T{u,v,x} = rank of jv in Ain[ex] if v < x (b-role bit jx of b), else rank
+//                of jx in Ain[b];      T{u,w,x} likewise with w, a;
+//     T{v,w,x}: (v,x) or (x,v) is the c-edge of T{u,v,x}; with lowest vertex v
+//                it is the rank of w's or x's position in the a-role row of the
+//                higher of the two edges from v; with lowest x (x < v) the rank
+//                of v's position in N+(x) in the a-role row of (x,w), the
+//                c-edge of T{u,x,w};
+//   down (x < u): T{x,u,v} and T{x,u,w} are the c-slots of bit i in Cm[b] and
+//                Cm[a]; T{x,v,w} is the c-slot of x's in-list position at v
+//                (inpos of (x,v), the a-edge of T{x,u,v}) in Cm[c].
+// This replaces the reference's K4 listing (cu_cliques_quds, compute_fourcliques,
+// nucleus/utils.cuh:550-619, 2173-2277), its triangle->K4 incidence sort
+// (nucleus/cuNucleus34.cu:609-673) and the on-demand re-enumeration by 3-way
+// list intersection (for_each_clique_of, nucleus/cuNucleus34_ondemand.cu:136-170):
+// one wave takes 64 triangles, flattens their row words over its lanes and
+// every lane expands the set bits of one word at a time.
 //
 // Peel (cuNucleus34_ondemand.cu:203-305, 599-690; cuNucleus34.cu:107-141):
 // level = max(level, min alive support); the triangles with support <= level
-// die this round (kv = level, round stamp). A K4 is destroyed by its first
-// member to die, same-round ties going to the lowest triangle index, and its
-// owner debits the surviving members. A survivor whose support falls from
-// level+1 to level is pushed onto the next round's frontier, so rounds after
-// the first of a level touch only the triangles that crossed.
+// die this round (kv = level). A K4 is destroyed by its first member to die,
+// same-round ties going to the lowest triangle index, and its owner debits the
+// surviving members. A survivor whose support falls from level+1 to level is
+// pushed onto the next round's frontier, so rounds after the first of a level
+// touch only the triangles that crossed.
 #include "cliques.hpp"
 
 namespace nuc34 {
@@ -25,178 +39,228 @@ namespace nuc34 {
 namespace {
 
 constexpr int kWPB = kBlock / kWave;
-constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-struct TriEdges {
-  uint32_t a, b, c;
+// ---- K4 enumeration ----------------------------------------------------------
+
+// A triangle's rows. Every Aout/Ain row of a vertex's out-edges has the same
+// length, so the row of (u, y) is rowA[first out-edge of u] + pos(y) * W.
+struct TriRows {
+  uint64_t ru0;     // Aout / Ain row of u's first out-edge (rows of a, b, (u,x) follow)
+  uint64_t rv0;     // Aout / Ain row of v's first out-edge
+  uint64_t ca, cb;  // Cm rows of a and b (over N-(u))
+  uint64_t cc;      // Cm row of c (over N-(v))
+  uint32_t c, ou, ov, jv, jw;
+  uint32_t W;       // words of the rows of u's out-edges
+  uint32_t Wv;      // words of the rows of v's out-edges
+  uint32_t WC;      // words of the a / b c-role rows
 };
 
-__device__ __forceinline__ TriEdges tri_edges(const TriView& v, uint64_t t) {
-  const uint64_t bc = v.triBC[t];
-  return {v.triA[t], static_cast<uint32_t>(bc >> 32), static_cast<uint32_t>(bc)};
+__device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
+  TriRows r;
+  const uint32_t a = s.triA[t];
+  const uint64_t bc = s.triBC[t];
+  const uint32_t b = static_cast<uint32_t>(bc >> 32);
+  r.c = static_cast<uint32_t>(bc);
+  r.ou = s.O[s.src[b]];
+  r.ov = s.O[s.E[b]];
+  r.jv = b - r.ou;
+  r.jw = a - r.ou;
+  const uint64_t ra = s.rows.rowA[a];
+  r.W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
+  r.ru0 = ra - static_cast<uint64_t>(r.jw) * r.W;
+  const uint64_t rc = s.rows.rowA[r.c];
+  r.Wv = static_cast<uint32_t>(s.rows.rowA[r.c + 1] - rc);
+  r.rv0 = rc - static_cast<uint64_t>(r.c - r.ov) * r.Wv;
+  r.ca = s.rows.rowC[a];
+  r.WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - r.ca);
+  r.cb = r.ca - static_cast<uint64_t>(r.jw) * r.WC + static_cast<uint64_t>(r.jv) * r.WC;
+  r.cc = s.rows.rowC[r.c];
+  return r;
 }
 
-__device__ __forceinline__ uint64_t tri_len(const TriView& v, const TriEdges& e) {
-  return (v.incO[e.a + 1] - v.incO[e.a]) + (v.incO[e.b + 1] - v.incO[e.b]) + (v.incO[e.c + 1] - v.incO[e.c]);
+// Word wi of t's K4 row (W up words, then WC down words) and the row words the
+// partner lookups of its bits use.
+struct K4Word {
+  uint32_t bits;
+  uint32_t Aa, Ab;  // up: b-role words of a and b
+  RankWord Ia, Ib;  // up: a-role words of a and b; down: c-role words of a and b
+};
+
+__device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint32_t wi) {
+  K4Word k;
+  if (wi < r.W) {
+    const uint64_t ra = r.ru0 + static_cast<uint64_t>(r.jw) * r.W + wi;
+    const uint64_t rb = r.ru0 + static_cast<uint64_t>(r.jv) * r.W + wi;
+    k.Aa = s.rows.Aout[ra];
+    k.Ab = s.rows.Aout[rb];
+    k.Ia = s.rows.Ain[ra];
+    k.Ib = s.rows.Ain[rb];
+    k.bits = (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
+  } else {
+    k.Aa = k.Ab = 0;
+    k.Ia = s.rows.Cm[r.ca + (wi - r.W)];
+    k.Ib = s.rows.Cm[r.cb + (wi - r.W)];
+    k.bits = k.Ia.bits & k.Ib.bits;
+  }
+  return k;
 }
 
-// x is the highest vertex of the partner from c's list exactly when c is that
-// partner's b-edge: then t holds the K4's lowest three vertices.
-__device__ __forceinline__ bool owns_k4(const TriView& v, uint32_t partner_c, uint32_t c) {
-  return static_cast<uint32_t>(v.triBC[partner_c] >> 32) == c;
-}
-
-// One lane: fn(partner via b, partner via a, partner via c) per K4, ascending x.
-template <typename F>
-__device__ __forceinline__ void k4_thread(const TriView& v, const TriEdges& e, F&& fn) {
-  uint64_t i = v.incO[e.b], iE = v.incO[e.b + 1];
-  uint64_t j = v.incO[e.a], jE = v.incO[e.a + 1];
-  uint64_t k = v.incO[e.c], kE = v.incO[e.c + 1];
-  while (i < iE && j < jE && k < kE) {
-    const uint32_t x1 = v.third[i], x2 = v.third[j], x3 = v.third[k];
-    if (x1 == x2 && x2 == x3) {
-      fn(v.tri[i], v.tri[j], v.tri[k]);
-      ++i;
-      ++j;
-      ++k;
+// The three partner triangles of the K4 at bit j of word wi.
+__device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, uint32_t wi, const K4Word& k,
+                                            uint32_t j, uint32_t q[3]) {
+  if (wi < r.W) {
+    const uint32_t jx = wi * 32 + j;
+    const bool vx = (k.Ab >> j) & 1u;  // v ranks below x
+    const bool wx = (k.Aa >> j) & 1u;  // w ranks below x
+    const RankWord* Ix = s.rows.Ain + r.ru0 + static_cast<uint64_t>(jx) * r.W;  // a-role row of (u, x)
+    q[0] = vx ? row_rank(Ix, r.jv) : rank_in_word(k.Ib, j);  // T{u,v,x}
+    q[1] = wx ? row_rank(Ix, r.jw) : rank_in_word(k.Ia, j);  // T{u,w,x}
+    const uint32_t evx = static_cast<uint32_t>(s.triBC[q[0]]);  // c-edge of T{u,v,x}: v-x
+    if (vx) {
+      const RankWord* Iv = s.rows.Ain + r.rv0;
+      q[2] = wx ? row_rank(Iv + static_cast<uint64_t>(evx - r.ov) * r.Wv, r.c - r.ov)  // v<w<x: a=(v,x), b=(v,w)
+                : row_rank(Iv + static_cast<uint64_t>(r.c - r.ov) * r.Wv, evx - r.ov);  // v<x<w: a=(v,w), b=(v,x)
     } else {
-      const uint32_t mx = max(x1, max(x2, x3));
-      i += (x1 < mx);
-      j += (x2 < mx);
-      k += (x3 < mx);
+      const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);  // x<v<w: a=(x,w), b=(x,v)
+      q[2] = row_rank(s.rows.Ain + s.rows.rowA[exw], evx - s.O[s.src[evx]]);
     }
+  } else {
+    q[0] = s.cIdx[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
+    q[1] = s.cIdx[rank_in_word(k.Ia, j)];  // T{x,u,w}: c-edge a
+    const uint32_t exv = s.triA[q[0]];     // a-edge of T{x,u,v}: (x,v)
+    q[2] = s.cIdx[row_rank(s.rows.Cm + r.cc, s.inpos[exv])];  // T{x,v,w}: c-edge c
   }
 }
 
-__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* s, uint32_t n, uint32_t key) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
+// Position of the r-th (from 0) set bit of x; x has more than r set bits.
+__device__ __forceinline__ uint32_t select_bit(uint32_t x, uint32_t r) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t w = 16; w >= 1; w >>= 1) {
+    const uint32_t lo = static_cast<uint32_t>(__popc(x & ((1u << w) - 1u)));
+    if (r >= lo) {
+      r -= lo;
+      x >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+// Largest index i < 64 with pre[i] <= key (pre non-decreasing, pre[0] = 0).
+__device__ __forceinline__ uint32_t find_slot(const uint32_t* pre, uint32_t key) {
+  uint32_t lo = 0, hi = kWave;
+  while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (s[mid] < key) lo = mid + 1;
+    if (pre[mid] <= key) lo = mid;
     else hi = mid;
   }
   return lo;
 }
 
-// Whole wave on one triangle: fn(match, index in b's list, in a's, in c's) is
-// called by every lane once per chunk step, in ascending x order across steps
-// and lanes.
-template <typename F>
-__device__ __forceinline__ void k4_wave(const TriView& v, const TriEdges& e, uint32_t* s1, uint32_t* s2, F&& fn) {
+// Per-wave LDS of the tile walk.
+struct TileLds {
+  TriRows rows[kWave];
+  uint32_t tri[kWave];
+  uint32_t pre[kWave + 1];  // first word item of each triangle
+  uint32_t run[kWave];      // K4s of each triangle listed so far
+  K4Word word[kWave];       // the current chunk of words ...
+  uint32_t wtri[kWave];     // ... their triangle slot,
+  uint32_t wi[kWave];       // ... word index,
+  uint32_t epre[kWave];     // ... first K4 in the chunk,
+  uint32_t lpos[kWave];     // ... and list position of their first K4
+};
+
+// One tile: lane l brings triangle t (when valid). The tile's row words are
+// flattened over the lanes 64 at a time, then the set bits of those 64 words
+// (the K4s) are flattened again, one K4 per lane and step, and emit(t, pos, q)
+// runs once per K4 of each triangle. With POS, pos is the K4's index in the
+// triangle's list (up words first, word and bit order); without, pos is 0.
+template <bool POS, typename Emit>
+__device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, TileLds& L, Emit&& emit) {
   const unsigned lane = lane_id();
-  uint64_t p0 = v.incO[e.b], e0 = v.incO[e.b + 1];
-  uint64_t p1 = v.incO[e.a], e1 = v.incO[e.a + 1];
-  uint64_t p2 = v.incO[e.c], e2 = v.incO[e.c + 1];
-  while (p0 < e0 && p1 < e1 && p2 < e2) {
-    const uint32_t n0 = static_cast<uint32_t>(e0 - p0 < kWave ? e0 - p0 : kWave);
-    const uint32_t n1 = static_cast<uint32_t>(e1 - p1 < kWave ? e1 - p1 : kWave);
-    const uint32_t n2 = static_cast<uint32_t>(e2 - p2 < kWave ? e2 - p2 : kWave);
-    const uint32_t x0 = lane < n0 ? v.third[p0 + lane] : kNone;
-    const uint32_t x1 = lane < n1 ? v.third[p1 + lane] : kNone;
-    const uint32_t x2 = lane < n2 ? v.third[p2 + lane] : kNone;
-    const uint32_t m0 = __shfl(x0, static_cast<int>(n0 - 1));
-    const uint32_t m1 = __shfl(x1, static_cast<int>(n1 - 1));
-    const uint32_t m2 = __shfl(x2, static_cast<int>(n2 - 1));
-    const uint32_t mm = min(m0, min(m1, m2));
-    s1[lane] = x1;
-    s2[lane] = x2;
+  uint32_t words = 0;
+  if (valid) {
+    const TriRows r = tri_rows(s, t);
+    L.rows[lane] = r;
+    words = r.W + r.WC;
+  }
+  L.tri[lane] = t;
+  L.run[lane] = 0;
+  const uint32_t inc = wave_inclusive_sum(words);
+  const uint32_t total = __shfl(inc, kWave - 1);
+  L.pre[lane] = inc - words;
+  if (lane == kWave - 1) L.pre[kWave] = inc;
+  wave_sync();
+  for (uint32_t base = 0; base < total; base += kWave) {
+    // 1. one word per lane
+    const uint32_t item = base + lane;
+    const bool act = item < total;
+    uint32_t i = 0, wi = 0;
+    K4Word k{};
+    if (act) {
+      i = find_slot(L.pre, item);
+      wi = item - L.pre[i];
+      k = k4_word(s, L.rows[i], wi);
+    }
+    const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
+    const uint32_t incl = wave_inclusive_sum(pc);
+    const uint32_t nk = __shfl(incl, kWave - 1);
+    L.word[lane] = k;
+    L.wtri[lane] = i;
+    L.wi[lane] = wi;
+    L.epre[lane] = incl - pc;
+    if constexpr (POS) {
+      // list position of the word's first K4: a segmented scan per triangle
+      const uint32_t first = act ? (L.pre[i] > base ? L.pre[i] - base : 0u) : lane;
+      const uint32_t before = __shfl(incl, static_cast<int>(first > 0 ? first - 1 : 0));
+      const uint32_t seg0 = first > 0 ? before : 0u;
+      L.lpos[lane] = act ? L.run[i] + incl - pc - seg0 : 0u;
+      wave_sync();
+      const bool last = act && (lane == kWave - 1 || item + 1 == total || item + 1 == L.pre[i + 1]);
+      if (last) L.run[i] += incl - seg0;
+    }
     wave_sync();
-    const bool take0 = lane < n0 && x0 <= mm;
-    bool match = false;
-    uint32_t j1 = 0, j2 = 0;
-    if (take0) {
-      j1 = lds_lower_bound(s1, n1, x0);
-      j2 = lds_lower_bound(s2, n2, x0);
-      match = j1 < n1 && s1[j1] == x0 && j2 < n2 && s2[j2] == x0;
+    // 2. one K4 per lane
+    for (uint32_t eb = 0; eb < nk; eb += kWave) {
+      const uint32_t e = eb + lane;
+      if (e < nk) {
+        const uint32_t sl = find_slot(L.epre, e);
+        const K4Word kw = L.word[sl];
+        const uint32_t r = e - L.epre[sl];
+        const uint32_t j = select_bit(kw.bits, r);
+        const uint32_t ti = L.wtri[sl];
+        uint32_t q[3];
+        k4_partners(s, L.rows[ti], L.wi[sl], kw, j, q);
+        emit(L.tri[ti], POS ? L.lpos[sl] + r : 0u, q);
+      }
     }
-    fn(match, p0 + lane, p1 + j1, p2 + j2);
-    p0 += static_cast<uint64_t>(__popcll(__ballot(take0)));
-    p1 += static_cast<uint64_t>(__popcll(__ballot(lane < n1 && x1 <= mm)));
-    p2 += static_cast<uint64_t>(__popcll(__ballot(lane < n2 && x2 <= mm)));
     wave_sync();
   }
 }
 
-// ---- support count ----------------------------------------------------------
+// ---- support, K4 lists, canonical K4 list -------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_support_small(TriView v, int32_t* __restrict__ sup,
-                                                          uint32_t* __restrict__ own, uint64_t* __restrict__ total,
-                                                          uint32_t* __restrict__ bigq,
-                                                          uint32_t* __restrict__ bigcount) {
-  uint64_t acc = 0;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < v.T; t += stride) {
-    const TriEdges e = tri_edges(v, t);
-    const bool big = tri_len(v, e) > kSmallL;
-    bool got;
-    const uint32_t slot = wave_append(bigcount, big, &got);
-    if (big) {
-      bigq[slot] = static_cast<uint32_t>(t);
-      continue;
-    }
-    int32_t cnt = 0;
-    uint32_t ow = 0;
-    k4_thread(v, e, [&](uint32_t, uint32_t, uint32_t pc) {
-      ++cnt;
-      ow += owns_k4(v, pc, e.c) ? 1u : 0u;
-    });
-    sup[t] = cnt;
-    own[t] = ow;
-    acc += static_cast<uint64_t>(cnt);
-  }
-  acc = wave_sum_u64(acc);
-  if (lane_id() == 0 && acc) atomic_add_u64(total, acc);
-}
-
-__global__ __launch_bounds__(kBlock) void k_support_big(TriView v, const uint32_t* __restrict__ bigq,
-                                                        const uint32_t* __restrict__ bigcount,
-                                                        int32_t* __restrict__ sup, uint32_t* __restrict__ own,
-                                                        uint64_t* __restrict__ total) {
-  __shared__ uint32_t s1[kWPB][kWave], s2[kWPB][kWave];
-  const unsigned w = threadIdx.x / kWave;
-  const uint32_t n = *bigcount;
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t q = blockIdx.x * kWPB + w; q < n; q += nw) {
-    const uint32_t t = bigq[q];
-    const TriEdges e = tri_edges(v, t);
-    uint32_t cnt = 0, ow = 0;
-    k4_wave(v, e, s1[w], s2[w], [&](bool match, uint64_t, uint64_t, uint64_t i2) {
-      cnt += static_cast<uint32_t>(__popcll(__ballot(match)));
-      const bool o = match && owns_k4(v, v.tri[i2], e.c);
-      ow += static_cast<uint32_t>(__popcll(__ballot(o)));
-    });
-    if (lane_id() == 0) {
-      sup[t] = static_cast<int32_t>(cnt);
-      own[t] = ow;
-      if (cnt) atomic_add_u64(total, cnt);
-    }
-  }
-}
-
-// ---- support and K4 lists from the bit rows --------------------------------
-
-__global__ __launch_bounds__(kBlock) void k_support_bits(const uint32_t* __restrict__ triA,
-                                                         const uint64_t* __restrict__ triBC, uint64_t T, BitRows r,
-                                                         int32_t* __restrict__ sup, uint32_t* __restrict__ sup_up,
+__global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __restrict__ sup,
                                                          uint32_t* __restrict__ own, uint64_t* __restrict__ total) {
   uint64_t acc = 0;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T; t += stride) {
-    const uint32_t a = triA[t];
-    const uint32_t b = static_cast<uint32_t>(triBC[t] >> 32);
-    const uint64_t ra = r.rowA[a], rb = r.rowA[b];
-    const uint32_t W = static_cast<uint32_t>(r.rowA[a + 1] - ra);
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < s.T; t += stride) {
+    const uint32_t a = s.triA[t];
+    const uint32_t b = static_cast<uint32_t>(s.triBC[t] >> 32);
+    const uint64_t ra = s.rows.rowA[a], rb = s.rows.rowA[b];
+    const uint32_t W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
     uint32_t up = 0, ow = 0, down = 0;
     for (uint32_t i = 0; i < W; ++i) {
-      const uint32_t oa = r.Aout[ra + i], ob = r.Aout[rb + i];
-      up += static_cast<uint32_t>(__popc((oa | r.Ain[ra + i]) & (ob | r.Ain[rb + i])));
+      const uint32_t oa = s.rows.Aout[ra + i], ob = s.rows.Aout[rb + i];
+      up += static_cast<uint32_t>(__popc((oa | s.rows.Ain[ra + i].bits) & (ob | s.rows.Ain[rb + i].bits)));
       ow += static_cast<uint32_t>(__popc(oa & ob));
     }
-    const uint64_t ca = r.rowC[a], cb = r.rowC[b];
-    const uint32_t WC = static_cast<uint32_t>(r.rowC[a + 1] - ca);
-    for (uint32_t i = 0; i < WC; ++i) down += static_cast<uint32_t>(__popc(r.Cm[ca + i] & r.Cm[cb + i]));
+    const uint64_t ca = s.rows.rowC[a], cb = s.rows.rowC[b];
+    const uint32_t WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - ca);
+    for (uint32_t i = 0; i < WC; ++i)
+      down += static_cast<uint32_t>(__popc(s.rows.Cm[ca + i].bits & s.rows.Cm[cb + i].bits));
     sup[t] = static_cast<int32_t>(up + down);
-    sup_up[t] = up;
     own[t] = ow;
     acc += up + down;
   }
@@ -204,25 +268,33 @@ __global__ __launch_bounds__(kBlock) void k_support_bits(const uint32_t* __restr
   if (lane_id() == 0 && acc) atomic_add_u64(total, acc);
 }
 
-__device__ __forceinline__ void put3(uint32_t* __restrict__ lists, uint64_t slot, uint32_t x, uint32_t y, uint32_t z) {
-  uint32_t* q = lists + 3 * slot;
-  q[0] = x;
-  q[1] = y;
-  q[2] = z;
+__global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __restrict__ koff,
+                                                     uint32_t* __restrict__ lists) {
+  __shared__ TileLds lds[kWPB];
+  const unsigned w = threadIdx.x / kWave;
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < s.T; tile += nw) {
+    const uint64_t t = tile * kWave + lane_id();
+    k4_tile<true>(s, static_cast<uint32_t>(t < s.T ? t : 0), t < s.T, lds[w],
+                  [&](uint32_t tt, uint32_t pos, const uint32_t q[3]) {
+                    uint32_t* o = lists + 3 * (koff[tt] + pos);
+                    o[0] = q[0];
+                    o[1] = q[1];
+                    o[2] = q[2];
+                  });
+  }
 }
 
-// Direct K4 lists, step 1: each triangle lists the K4s it owns (fourth vertex
-// above all three of its vertices) as (t << 32 | position of x in N+(u)).
-__global__ __launch_bounds__(kBlock) void k_k4_owned(const uint32_t* __restrict__ triA,
-                                                     const uint64_t* __restrict__ triBC, uint64_t T, BitRows r,
-                                                     const uint64_t* __restrict__ k4off, uint64_t* __restrict__ k4) {
+// Owned K4s: fourth vertex above all of t, i.e. b-role bits of both a and b.
+__global__ __launch_bounds__(kBlock) void k_k4_owned(K4Src s, const uint64_t* __restrict__ k4off,
+                                                     uint64_t* __restrict__ k4) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T; t += stride) {
-    const uint32_t a = triA[t];
-    const uint32_t b = static_cast<uint32_t>(triBC[t] >> 32);
-    const uint32_t* Aa = r.Aout + r.rowA[a];
-    const uint32_t* Ab = r.Aout + r.rowA[b];
-    const uint32_t W = static_cast<uint32_t>(r.rowA[a + 1] - r.rowA[a]);
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < s.T; t += stride) {
+    const uint32_t a = s.triA[t];
+    const uint32_t b = static_cast<uint32_t>(s.triBC[t] >> 32);
+    const uint32_t* Aa = s.rows.Aout + s.rows.rowA[a];
+    const uint32_t* Ab = s.rows.Aout + s.rows.rowA[b];
+    const uint32_t W = static_cast<uint32_t>(s.rows.rowA[a + 1] - s.rows.rowA[a]);
     uint64_t pos = k4off[t];
     for (uint32_t wi = 0; wi < W; ++wi) {
       uint32_t bits = Aa[wi] & Ab[wi];
@@ -235,85 +307,24 @@ __global__ __launch_bounds__(kBlock) void k_k4_owned(const uint32_t* __restrict_
   }
 }
 
-// The four triangles of the K4 {u < v < w < x} owned by t = T(u,v,w), x at
-// position jx of N+(u): T(u,v,x) and T(u,w,x) from the a-role rows of (u,x),
-// T(v,w,x) from the a-role row of (v,x) = c-edge of T(u,v,x).
-struct K4Ids {
-  uint32_t t, t1, t2, t3;  // T(u,v,w), T(u,v,x), T(u,w,x), T(v,w,x)
-  uint32_t a, b, c, c1;    // edges (u,w), (u,v), (v,w), (v,x)
-  uint32_t ou, jv, jw, ex;
-};
-
-__device__ __forceinline__ K4Ids k4_ids(uint64_t key, const uint32_t* __restrict__ triA,
-                                        const uint64_t* __restrict__ triBC, const uint32_t* __restrict__ O,
-                                        const uint32_t* __restrict__ E, const uint32_t* __restrict__ src,
-                                        const BitRows& r, const uint64_t* __restrict__ offA) {
-  K4Ids k;
-  k.t = static_cast<uint32_t>(key >> 32);
-  const uint32_t jx = static_cast<uint32_t>(key);
-  k.a = triA[k.t];
-  const uint64_t bc = triBC[k.t];
-  k.b = static_cast<uint32_t>(bc >> 32);
-  k.c = static_cast<uint32_t>(bc);
-  k.ou = O[src[k.b]];
-  k.jv = k.b - k.ou;
-  k.jw = k.a - k.ou;
-  k.ex = k.ou + jx;
-  const uint32_t* Ix = r.Ain + r.rowA[k.ex];
-  k.t1 = static_cast<uint32_t>(offA[k.ex] + bits_below(Ix, k.jv));
-  k.t2 = static_cast<uint32_t>(offA[k.ex] + bits_below(Ix, k.jw));
-  k.c1 = static_cast<uint32_t>(triBC[k.t1]);
-  const uint32_t ov = O[E[k.b]];
-  k.t3 = static_cast<uint32_t>(offA[k.c1] + bits_below(r.Ain + r.rowA[k.c1], k.c - ov));
-  return k;
-}
-
-// Direct K4 lists, step 2: one thread per K4; an entry (the other three
-// triangle ids) in the list of each of its four triangles, at a position fixed
-// by prefix counts: up-K4s (fourth vertex above the triangle's lowest vertex)
-// first, in fourth-vertex order, then down-K4s in order of the lowest vertex's
-// in-list position.
-__global__ __launch_bounds__(kBlock) void k_k4_place(const uint64_t* __restrict__ k4, uint64_t Q,
-                                                     const uint32_t* __restrict__ triA,
-                                                     const uint64_t* __restrict__ triBC,
-                                                     const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
-                                                     const uint32_t* __restrict__ src,
-                                                     const uint32_t* __restrict__ inpos, BitRows r,
-                                                     const uint64_t* __restrict__ offA,
-                                                     const uint64_t* __restrict__ koff,
-                                                     const uint32_t* __restrict__ sup_up,
-                                                     uint32_t* __restrict__ lists) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < Q; q += stride) {
-    const K4Ids k = k4_ids(k4[q], triA, triBC, O, E, src, r, offA);
-    const uint32_t* Aa = r.Aout + r.rowA[k.a];
-    const uint32_t* Ab = r.Aout + r.rowA[k.b];
-    const uint32_t* Ia = r.Ain + r.rowA[k.a];
-    const uint32_t* Ib = r.Ain + r.rowA[k.b];
-    const uint32_t* Ax = r.Aout + r.rowA[k.ex];
-    const uint32_t* Ix = r.Ain + r.rowA[k.ex];
-    const uint32_t jx = k.ex - k.ou;
-    put3(lists, koff[k.t] + sym_bits_below(Aa, Ia, Ab, Ib, jx), k.t1, k.t2, k.t3);
-    put3(lists, koff[k.t1] + sym_bits_below(Ab, Ib, Ax, Ix, k.jw), k.t, k.t2, k.t3);
-    put3(lists, koff[k.t2] + sym_bits_below(Aa, Ia, Ax, Ix, k.jv), k.t, k.t1, k.t3);
-    put3(lists,
-         koff[k.t3] + sup_up[k.t3] + and_bits_below(r.Cm + r.rowC[k.c], r.Cm + r.rowC[k.c1], inpos[k.b]), k.t,
-         k.t1, k.t2);
-  }
-}
-
-__global__ void k_k4_quads(const uint64_t* __restrict__ k4, uint64_t first, uint64_t count,
-                           const uint32_t* __restrict__ triA, const uint64_t* __restrict__ triBC,
-                           const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
-                           const uint32_t* __restrict__ src, BitRows r, const uint64_t* __restrict__ offA,
+// {T(pqs), T(pqr), T(prs), T(qrs)} of the owned K4 (t = T(u,v,w), x at
+// position jx of N+(u), u<v<w<x).
+__global__ void k_k4_quads(K4Src s, const uint64_t* __restrict__ k4, uint64_t first, uint64_t count,
                            uint32_t* __restrict__ quads) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const K4Ids k = k4_ids(k4[first + i], triA, triBC, O, E, src, r, offA);
-    quads[4 * i + 0] = k.t1;
-    quads[4 * i + 1] = k.t;
-    quads[4 * i + 2] = k.t2;
-    quads[4 * i + 3] = k.t3;
+    const uint64_t key = k4[first + i];
+    const uint32_t t = static_cast<uint32_t>(key >> 32);
+    const TriRows r = tri_rows(s, t);
+    const RankWord* Ix = s.rows.Ain + r.ru0 + static_cast<uint64_t>(static_cast<uint32_t>(key)) * r.W;
+    const uint32_t t1 = row_rank(Ix, r.jv);  // T(u,v,x)
+    const uint32_t t2 = row_rank(Ix, r.jw);  // T(u,w,x)
+    const uint32_t evx = static_cast<uint32_t>(s.triBC[t1]);
+    const uint32_t t3 = row_rank(s.rows.Ain + r.rv0 + static_cast<uint64_t>(evx - r.ov) * r.Wv, r.c - r.ov);
+    quads[4 * i + 0] = t1;
+    quads[4 * i + 1] = t;
+    quads[4 * i + 2] = t2;
+    quads[4 * i + 3] = t3;
   }
 }
 
@@ -380,7 +391,6 @@ __global__ void k_peel_ctl_init(PeelCtl* ctl) {
     c.level = 0;
     c.round = 0;
     c.done = 0;
-    c.big = 0;
     c.error = 0;
     c.peeled = 0;
   }
@@ -534,7 +544,6 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
     out->need_level = need ? 1u : 0u;
     out->level = round_level(in);
     out->round = in->round + 1;
-    out->big = 0;
     out->error = error;
     out->peeled = peeled;
     out->done = done;
@@ -567,101 +576,58 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_peel_debit_od_small(PeelBufs p, PeelCtl* in, TriView v) {
+// On-demand debit: the frontier in tiles of 64, each dying triangle's K4s
+// enumerated from the bit rows.
+__global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, K4Src s) {
+  __shared__ TileLds lds[kWPB];
   if (in->done) return;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   uint32_t* ncount = &in->nnext;
   const uint32_t n = in->ncur;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = cur[i];
-    const TriEdges e = tri_edges(v, t);
-    const bool big = tri_len(v, e) > kSmallL;
-    bool got;
-    const uint32_t slot = wave_append(&in->big, big, &got);
-    if (big) {
-      p.bigq[slot] = t;
-      continue;
-    }
-    k4_thread(v, e, [&](uint32_t pb, uint32_t pa, uint32_t pcc) {
-      const uint32_t qq[3] = {pb, pa, pcc};
-      destroy_k4(t, qq, p, level, nxt, ncount);
-    });
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_peel_debit_od_big(PeelBufs p, PeelCtl* in, TriView v) {
-  __shared__ uint32_t s1[kWPB][kWave], s2[kWPB][kWave];
-  if (in->done) return;
-  const int32_t level = round_level(in);
-  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
-  uint32_t* ncount = &in->nnext;
-  const uint32_t n = in->big;
   const unsigned w = threadIdx.x / kWave;
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t q = blockIdx.x * kWPB + w; q < n; q += nw) {
-    const uint32_t t = p.bigq[q];
-    const TriEdges e = tri_edges(v, t);
-    k4_wave(v, e, s1[w], s2[w], [&](bool match, uint64_t i0, uint64_t i1, uint64_t i2) {
-      if (match) {
-        const uint32_t qq[3] = {v.tri[i0], v.tri[i1], v.tri[i2]};
-        destroy_k4(t, qq, p, level, nxt, ncount);
-      }
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < n; tile += nw) {
+    const uint64_t i = tile * kWave + lane_id();
+    const bool valid = i < n;
+    k4_tile<false>(s, valid ? cur[i] : 0u, valid, lds[w], [&](uint32_t tt, uint32_t, const uint32_t q[3]) {
+      destroy_k4(tt, q, p, level, nxt, ncount);
     });
   }
 }
 
 constexpr unsigned kRoundGrid = 2048;
+constexpr unsigned kScanGrid = 512;
 
 }  // namespace
 
-void launch_support(const TriView& v, int32_t* support, uint32_t* owned, uint64_t* total, uint32_t* bigq,
-                    uint32_t* bigcount, hipStream_t s) {
-  if (!v.T) return;
-  hipLaunchKernelGGL(k_support_small, dim3(grid_for(v.T, kBlock, 1u << 14)), dim3(kBlock), 0, s, v, support, owned,
-                     total, bigq, bigcount);
-  NUC34_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_support_big, dim3(8192), dim3(kBlock), 0, s, v, bigq, bigcount, support, owned, total);
+void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st) {
+  if (!s.T) return;
+  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(s.T, kBlock, 1u << 14)), dim3(kBlock), 0, st, s, sup, own, total);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_support_bits(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows, int32_t* sup,
-                         uint32_t* sup_up, uint32_t* own, uint64_t* total, hipStream_t s) {
-  if (!T) return;
-  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(T, kBlock, 1u << 14)), dim3(kBlock), 0, s, triA, triBC, T, rows,
-                     sup, sup_up, own, total);
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, hipStream_t st) {
+  if (!s.T) return;
+  const uint64_t tiles = (s.T + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for(tiles, kWPB, 1u << 15)), dim3(kBlock), 0, st, s, koff, lists);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_owned(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows,
-                     const uint64_t* k4off, uint64_t* k4, hipStream_t s) {
-  if (!T) return;
-  hipLaunchKernelGGL(k_k4_owned, dim3(grid_for(T, kBlock, 1u << 14)), dim3(kBlock), 0, s, triA, triBC, T, rows, k4off,
-                     k4);
+void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t* k4, hipStream_t st) {
+  if (!s.T) return;
+  hipLaunchKernelGGL(k_k4_owned, dim3(grid_for(s.T, kBlock, 1u << 14)), dim3(kBlock), 0, st, s, k4off, k4);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_place(const uint64_t* k4, uint64_t Q, const uint32_t* triA, const uint64_t* triBC, const uint32_t* O,
-                     const uint32_t* E, const uint32_t* src, const uint32_t* inpos, const BitRows& rows,
-                     const uint64_t* offA, const uint64_t* koff, const uint32_t* sup_up, uint32_t* lists,
-                     hipStream_t s) {
-  if (!Q) return;
-  hipLaunchKernelGGL(k_k4_place, dim3(grid_for(Q, kBlock, 1u << 16)), dim3(kBlock), 0, s, k4, Q, triA, triBC, O, E,
-                     src, inpos, rows, offA, koff, sup_up, lists);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, const uint32_t* triA,
-                     const uint64_t* triBC, const uint32_t* O, const uint32_t* E, const uint32_t* src,
-                     const BitRows& rows, const uint64_t* offA, uint32_t* quads, hipStream_t s) {
+void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, uint64_t count, uint32_t* quads,
+                     hipStream_t st) {
   if (!count) return;
-  hipLaunchKernelGGL(k_k4_quads, dim3(grid_for(count, 256, 1u << 14)), dim3(256), 0, s, k4owned, first, count, triA,
-                     triBC, O, E, src, rows, offA, quads);
+  hipLaunchKernelGGL(k_k4_quads, dim3(grid_for(count, 256, 1u << 14)), dim3(256), 0, st, s, k4owned, first, count,
+                     quads);
   NUC34_LAUNCH_CHECK();
 }
-
-constexpr unsigned kScanGrid = 512;
 
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, p.ctl);
@@ -679,12 +645,11 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_peel_round_ondemand(const PeelBufs& p, int slot, const TriView& v, hipStream_t s) {
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
   hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
-  hipLaunchKernelGGL(k_peel_debit_od_small, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, v);
-  hipLaunchKernelGGL(k_peel_debit_od_big, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, v);
+  hipLaunchKernelGGL(k_peel_debit_bits, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, src);
   hipLaunchKernelGGL(k_peel_finish, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out);
   NUC34_LAUNCH_CHECK();
 }

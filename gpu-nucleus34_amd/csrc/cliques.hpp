@@ -1,51 +1,41 @@
-// (c), (d), (e): four-clique support, four-clique list and triangle->K4
-// incidence (direct), and the level-synchronous peel with frontiers.
+// (c), (d), (e): four-clique support, the per-triangle K4 lists (direct) and
+// the level-synchronous peel, all driven by one K4 enumeration from the ranked
+// bit rows (bitrows.hpp).
 #pragma once
 #include "bitrows.hpp"
 #include "common.hpp"
 
 namespace nuc34 {
 
-// Read-only view of the triangle list and its incidence.
-struct TriView {
+// Everything a K4 enumeration from the bit rows reads.
+struct K4Src {
+  const uint32_t* O;      // oriented CSR offsets (n + 1)
+  const uint32_t* E;      // edge heads (m)
+  const uint32_t* src;    // edge tails (m)
+  const uint32_t* inpos;  // position of each edge in its head's in-list
+  BitRows rows;
   const uint32_t* triA;   // a-edge per triangle
   const uint64_t* triBC;  // (b << 32) | c per triangle
+  const uint32_t* cIdx;   // triangle id per c-slot
   uint64_t T;
-  const uint64_t* incO;   // m + 1
-  const uint32_t* third;  // 3T
-  const uint32_t* tri;    // 3T
 };
 
-// Triangles whose three incidence lists hold more than this many entries in
-// total are intersected by a whole wave; shorter ones by one lane.
-constexpr uint32_t kSmallL = 128;
-
-// support[t] = K4s on t, owned[t] = K4s whose lowest three vertices are t,
-// *total += sum of supports. Triangles above kSmallL are appended to bigq.
-void launch_support(const TriView& v, int32_t* support, uint32_t* owned, uint64_t* total, uint32_t* bigq,
-                    uint32_t* bigcount, hipStream_t s);
-// Support from the bit rows: sup = K4s on t, sup_up = those whose fourth vertex
-// ranks above t's lowest vertex, own = those whose fourth vertex ranks above all
-// of t (t holds the K4's lowest three vertices); *total += sum of sup.
-void launch_support_bits(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows, int32_t* sup,
-                         uint32_t* sup_up, uint32_t* own, uint64_t* total, hipStream_t s);
-// Direct variant: per-triangle K4 lists, three partner triangle ids per entry,
-// at lists[3*koff[t] ..]. Each K4 is enumerated once, from the triangle on its
-// lowest three vertices (k4owned: (t << 32 | x position) at k4off[t] from a scan
-// of `owned`), and written into the lists of its four triangles at positions
-// given by prefix counts (up-K4s first, in fourth-vertex order, then down-K4s).
-void launch_k4_owned(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const BitRows& rows,
-                     const uint64_t* k4off, uint64_t* k4owned, hipStream_t s);
-void launch_k4_place(const uint64_t* k4owned, uint64_t Q, const uint32_t* triA, const uint64_t* triBC,
-                     const uint32_t* O, const uint32_t* E, const uint32_t* src, const uint32_t* inpos,
-                     const BitRows& rows, const uint64_t* offA, const uint64_t* koff, const uint32_t* sup_up,
-                     uint32_t* lists, hipStream_t s);
+// sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
+// (t holds the K4's lowest three vertices); *total += sum of sup.
+void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st);
+// Direct variant: the K4 list of every triangle, three partner triangle ids per
+// entry, at lists[3 * koff[t] ..] (koff = exclusive scan of sup): the K4s whose
+// fourth vertex ranks above t's lowest vertex first, in order of its position
+// in that vertex's out-list, then the others in order of their fourth vertex's
+// position in the lowest vertex's in-list.
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, hipStream_t st);
+// Owned K4s: (t << 32 | position of the fourth vertex in N+(u)) at k4off[t] ..
+void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t* k4owned, hipStream_t st);
 // The K4 list in canonical order (by the triangle on the three lowest vertices,
 // then by fourth vertex) as {T(pqs), T(pqr), T(prs), T(qrs)} for p<q<r<s, for
 // owned-K4 records [first, first + count).
-void launch_k4_quads(const uint64_t* k4owned, uint64_t first, uint64_t count, const uint32_t* triA,
-                     const uint64_t* triBC, const uint32_t* O, const uint32_t* E, const uint32_t* src,
-                     const BitRows& rows, const uint64_t* offA, uint32_t* quads, hipStream_t s);
+void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, uint64_t count, uint32_t* quads,
+                     hipStream_t st);
 
 // ---- peel ------------------------------------------------------------------
 //
@@ -72,8 +62,8 @@ struct PeelCtl {
   int32_t level;
   uint32_t round;
   uint32_t done;
-  uint32_t big;         // frontier triangles queued for the wave path (on-demand)
   uint32_t error;       // nonzero: inconsistent state detected
+  uint32_t pad;
   uint64_t peeled;
 };
 
@@ -83,7 +73,6 @@ struct PeelBufs {
   int32_t* kv;
   uint32_t* buf0;   // frontier buffers, T entries each
   uint32_t* buf1;
-  uint32_t* bigq;   // on-demand: frontier triangles for the wave path
   PeelCtl* ctl;     // two slots
   uint64_t T;
 };
@@ -91,11 +80,11 @@ struct PeelBufs {
 // One peel round at position `slot` of its group: level selection when the
 // frontier is empty, the debit of every K4 owned by a dying triangle, and the
 // state advance. Every kernel returns at once when the round's control says
-// done, so extra rounds are no-ops.
+// done, so extra rounds are no-ops. The direct round reads the K4 lists; the
+// on-demand round enumerates each dying triangle's K4s from the bit rows.
 void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
                               hipStream_t s);
-void launch_peel_round_ondemand(const PeelBufs& p, int slot, const TriView& v, hipStream_t s);
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
-
 
 }  // namespace nuc34

@@ -13,18 +13,13 @@ void launch_orient_fill(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, cons
 void launch_fill_src(const uint32_t* O, uint32_t n, uint32_t* src, hipStream_t s);
 
 // (b) triangle enumeration over oriented edges [e_begin, e_end): the bits pass
-// fills the incidence rows, the emit pass writes triangles at canonical indices.
+// fills the incidence rows, the emit pass (after the row ranks) writes the
+// triangles at their canonical indices and their ids at their c-slots.
 void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
                      const BitRows& rows, const uint32_t* inpos, hipStream_t s);
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint64_t* offA, uint32_t* triA, uint64_t* triBC, hipStream_t s);
-
-// (d) edge -> triangle incidence.
-void launch_inc_len(const uint32_t* cntA, const uint32_t* cntB, const uint32_t* cntC, uint64_t m, uint32_t* len,
-                    hipStream_t s);
-void launch_inc_fill(const uint32_t* triA, const uint64_t* triBC, uint64_t T, const uint32_t* E,
-                     const uint32_t* src, const uint64_t* incO, uint32_t* cur, uint64_t* inc, hipStream_t s);
-void launch_inc_split(const uint64_t* inc, uint64_t n, uint32_t* third, uint32_t* tri, hipStream_t s);
+                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
+                     hipStream_t s);
 
 // out[i] = src[idx[i]] for i < n (idx in bounds of src, checked by the caller).
 void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s);

@@ -13,10 +13,12 @@
 //   bits pass: every triangle sets one bit in each of three per-edge rows (the
 //              bit-packed edge->triangle incidence, bitrows.hpp): the b-role row
 //              of b, the a-role row of a and the c-role row of c;
-//   emit pass: a triangle's canonical index is offA[a] plus the number of
-//              a-role bits of a below v's position, i.e. exactly its place in the
-//              reference's (a, b)-sorted triangle list (tri_t::operator<,
-//              utils.cuh:93-102), with no sort and no atomics.
+//   emit pass: a triangle's canonical index is the rank of v's position in the
+//              a-role row of a (offA[a] plus the a-role bits below it), i.e.
+//              exactly its place in the reference's (a, b)-sorted triangle list
+//              (tri_t::operator<, utils.cuh:93-102), with no sort and no
+//              atomics; it is also stored at its c-slot (cIdx), the rank of the
+//              lowest vertex's in-list position in the c-role row of c.
 #include "kernels.hpp"
 
 namespace nuc34 {
@@ -41,8 +43,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ src, uint64_t e_begin,
                                                      uint64_t e_end, BitRows rows,
                                                      const uint32_t* __restrict__ inpos,
-                                                     const uint64_t* __restrict__ offA,
-                                                     uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC) {
+                                                     uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
+                                                     uint32_t* __restrict__ cIdx) {
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
   __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
   __shared__ uint32_t s_len[kWavesPerBlock][kWave];
@@ -114,13 +116,14 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
             // b-role: w seen from (u,v); a-role: v seen from (u,w); c-role: u's
             // position among the in-neighbours of v, seen from (v,w).
             atomicOr(&rows.Aout[rows.rowA[b] + (jw >> 5)], 1u << (jw & 31));
-            atomicOr(&rows.Ain[rows.rowA[a] + (jv >> 5)], 1u << (jv & 31));
+            atomicOr(&rows.Ain[rows.rowA[a] + (jv >> 5)].bits, 1u << (jv & 31));
             const uint32_t iu = inpos[b];
-            atomicOr(&rows.Cm[rows.rowC[c] + (iu >> 5)], 1u << (iu & 31));
+            atomicOr(&rows.Cm[rows.rowC[c] + (iu >> 5)].bits, 1u << (iu & 31));
           } else {
-            const uint64_t p = offA[a] + bits_below(rows.Ain + rows.rowA[a], jv);
+            const uint32_t p = row_rank(rows.Ain + rows.rowA[a], jv);
             triA[p] = a;
             triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
+            cIdx[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = p;
           }
         }
       }
@@ -145,10 +148,11 @@ void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, 
 }
 
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint64_t* offA, uint32_t* triA, uint64_t* triBC, hipStream_t s) {
+                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
+                     hipStream_t s) {
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, nullptr, offA, triA, triBC);
+                     e_end, rows, inpos, triA, triBC, cIdx);
   NUC34_LAUNCH_CHECK();
 }
 
