@@ -43,6 +43,7 @@ struct nuc34_ctx {
   uint64_t T = 0;
   DevBuf IO, IE, IEe, inpos;        // in-CSR of the oriented graph
   DevBuf rowA, rowC, Aout, Ain, Cm;  // bit rows
+  uint64_t WA = 0, WC = 0;           // words of the Aout/Ain rows and of the Cm rows
   DevBuf offA, triA, triBC;
   DevBuf cIdx;                       // triangle id per c-slot
   // support
@@ -231,7 +232,7 @@ BitRows bitrows(nuc34_ctx* c) {
 K4Src k4src(nuc34_ctx* c) {
   return K4Src{c->O.as<uint32_t>(),    c->E.as<uint32_t>(),     c->src.as<uint32_t>(), c->inpos.as<uint32_t>(),
                bitrows(c),             c->triA.as<uint32_t>(),  c->triBC.as<uint64_t>(), c->cIdx.as<uint32_t>(),
-               c->T};
+               c->T,                   nullptr,                 nullptr};
 }
 
 void run_triangles(nuc34_ctx* c, uint32_t parts) {
@@ -258,8 +259,8 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
       exclusive_scan<uint32_t, uint64_t>(wa.as<uint32_t>(), c->rowA.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
       exclusive_scan<uint32_t, uint64_t>(wc.as<uint32_t>(), c->rowC.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
     }
-    const uint64_t WA = read_scalar<uint64_t>(c, c->rowA.as<uint64_t>() + m);
-    const uint64_t WC = read_scalar<uint64_t>(c, c->rowC.as<uint64_t>() + m);
+    const uint64_t WA = c->WA = read_scalar<uint64_t>(c, c->rowA.as<uint64_t>() + m);
+    const uint64_t WC = c->WC = read_scalar<uint64_t>(c, c->rowC.as<uint64_t>() + m);
     c->Aout = c->alloc.get(WA * 4 + 4, "b-role rows");
     c->Ain = c->alloc.get(WA * 8 + 8, "a-role rows");
     c->Cm = c->alloc.get(WC * 8 + 8, "c-role rows");
@@ -366,7 +367,14 @@ void run_peel(nuc34_ctx* c, int variant) {
   DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
               b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
-  const K4Src v = k4src(c);
+  K4Src v = k4src(c);
+  DevBuf liveA, liveC;
+  if (variant != NUC34_DIRECT) {
+    liveA = c->alloc.get(c->WA * 4 + 4, "live a/b rows");
+    liveC = c->alloc.get(c->WC * 4 + 4, "live c rows");
+    v.LA = liveA.as<uint32_t>();
+    v.LC = liveC.as<uint32_t>();
+  }
   static_assert(kRoundsPerLaunch % 2 == 0, "round parity must equal slot parity");
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {
@@ -380,6 +388,7 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl h{};
   c->st.ms_peel = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
+    if (variant != NUC34_DIRECT) launch_live_rows(v, c->WA, c->WC, c->s);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {

@@ -86,6 +86,8 @@ struct K4Word {
   RankWord Ia, Ib;  // up: a-role words of a and b; down: c-role words of a and b
 };
 
+// LIVE: the K4 bits come from the live rows (the on-demand peel).
+template <bool LIVE>
 __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint32_t wi) {
   K4Word k;
   if (wi < r.W) {
@@ -95,12 +97,13 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     k.Ab = s.rows.Aout[rb];
     k.Ia = s.rows.Ain[ra];
     k.Ib = s.rows.Ain[rb];
-    k.bits = (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
+    k.bits = LIVE ? s.LA[ra] & s.LA[rb] : (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
   } else {
+    const uint64_t ca = r.ca + (wi - r.W), cb = r.cb + (wi - r.W);
     k.Aa = k.Ab = 0;
-    k.Ia = s.rows.Cm[r.ca + (wi - r.W)];
-    k.Ib = s.rows.Cm[r.cb + (wi - r.W)];
-    k.bits = k.Ia.bits & k.Ib.bits;
+    k.Ia = s.rows.Cm[ca];
+    k.Ib = s.rows.Cm[cb];
+    k.bits = LIVE ? s.LC[ca] & s.LC[cb] : k.Ia.bits & k.Ib.bits;
   }
   return k;
 }
@@ -201,7 +204,7 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
     if (act) {
       i = find_slot(L.pre, item);
       wi = item - L.pre[i];
-      k = k4_word(s, L.rows[i], wi);
+      k = k4_word<false>(s, L.rows[i], wi);
     }
     const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
     const uint32_t incl = wave_inclusive_sum(pc);
@@ -233,6 +236,48 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
         uint32_t q[3];
         k4_partners(s, L.rows[ti], L.wi[sl], kw, j, q);
         emit(L.tri[ti], POS ? L.lpos[sl] + r : 0u, q);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// Per-wave LDS of the single-triangle walk.
+struct OneLds {
+  K4Word word[kWave];
+  uint32_t wi[kWave];
+  uint32_t epre[kWave];
+};
+
+// One triangle on a whole wave (the on-demand peel, live rows): its row words
+// 64 at a time, then their set bits (the K4s) one per lane and step; emit(q)
+// once per K4. Every lane calls it with the same t.
+template <typename Emit>
+__device__ __forceinline__ void k4_one(const K4Src& s, uint32_t t, OneLds& L, Emit&& emit) {
+  const unsigned lane = lane_id();
+  const TriRows r = tri_rows(s, t);
+  const uint32_t words = r.W + r.WC;
+  for (uint32_t base = 0; base < words; base += kWave) {
+    const uint32_t wi = base + lane;
+    K4Word k{};
+    if (wi < words) k = k4_word<true>(s, r, wi);
+    const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
+    const uint32_t incl = wave_inclusive_sum(pc);
+    const uint32_t nk = __shfl(incl, kWave - 1);
+    if (!nk) continue;  // wave-uniform
+    L.word[lane] = k;
+    L.wi[lane] = wi;
+    L.epre[lane] = incl - pc;
+    wave_sync();
+    for (uint32_t eb = 0; eb < nk; eb += kWave) {
+      const uint32_t e = eb + lane;
+      if (e < nk) {
+        const uint32_t sl = find_slot(L.epre, e);
+        const K4Word kw = L.word[sl];
+        const uint32_t j = select_bit(kw.bits, e - L.epre[sl]);
+        uint32_t q[3];
+        k4_partners(s, r, L.wi[sl], kw, j, q);
+        emit(q);
       }
     }
     wave_sync();
@@ -509,8 +554,28 @@ __device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], cons
 // the pushed become the next frontier); when nothing was pushed, the min alive
 // support for the next level goes into out->minval. Workgroup 0 writes the
 // next round's control into `out`; no other workgroup reads `out`.
-__global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out) {
+// Clears the live-row bits of this round's dying triangles (on-demand only;
+// after the debit, so same-round ownership still sees every K4).
+__device__ __forceinline__ void unlink_frontier(const PeelBufs& p, const PeelCtl* in, const K4Src& s) {
+  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  const uint32_t n = in->ncur;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t t = cur[i];
+    const uint32_t a = s.triA[t];
+    const uint64_t bc = s.triBC[t];
+    const uint32_t b = static_cast<uint32_t>(bc >> 32), c = static_cast<uint32_t>(bc);
+    const uint32_t ou = s.O[s.src[b]];
+    const uint32_t jv = b - ou, jw = a - ou, ip = s.inpos[b];
+    atomic_and_u32(&s.LA[s.rows.rowA[a] + (jv >> 5)], ~(1u << (jv & 31)));  // t in a's row
+    atomic_and_u32(&s.LA[s.rows.rowA[b] + (jw >> 5)], ~(1u << (jw & 31)));  // t in b's row
+    atomic_and_u32(&s.LC[s.rows.rowC[c] + (ip >> 5)], ~(1u << (ip & 31)));  // t in c's c-role row
+  }
+}
+
+template <bool UNLINK>
+__global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
   if (in->done) return;
+  if (UNLINK) unlink_frontier(p, in, s);
   const bool need = in->nnext == 0;
   const uint64_t W = (p.T + 15) / 16;
   uint32_t mn = kNoMin;
@@ -576,10 +641,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   }
 }
 
-// On-demand debit: the frontier in tiles of 64, each dying triangle's K4s
-// enumerated from the bit rows.
+// On-demand debit: one wave per dying triangle, its K4s enumerated from the
+// bit rows (rounds with few dying triangles still spread over many waves).
 __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, K4Src s) {
-  __shared__ TileLds lds[kWPB];
+  __shared__ OneLds lds[kWPB];
   if (in->done) return;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
@@ -587,14 +652,19 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   uint32_t* ncount = &in->nnext;
   const uint32_t n = in->ncur;
   const unsigned w = threadIdx.x / kWave;
-  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < n; tile += nw) {
-    const uint64_t i = tile * kWave + lane_id();
-    const bool valid = i < n;
-    k4_tile<false>(s, valid ? cur[i] : 0u, valid, lds[w], [&](uint32_t tt, uint32_t, const uint32_t q[3]) {
-      destroy_k4(tt, q, p, level, nxt, ncount);
-    });
+  const uint32_t nw = gridDim.x * kWPB;
+  for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
+    const uint32_t t = cur[i];
+    k4_one(s, t, lds[w], [&](const uint32_t q[3]) { destroy_k4(t, q, p, level, nxt, ncount); });
   }
+}
+
+__global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < WA; i += stride)
+    s.LA[i] = s.rows.Aout[i] | s.rows.Ain[i].bits;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < WC; i += stride)
+    s.LC[i] = s.rows.Cm[i].bits;
 }
 
 constexpr unsigned kRoundGrid = 2048;
@@ -629,6 +699,12 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
   NUC34_LAUNCH_CHECK();
 }
 
+void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s) {
+  if (!WA && !WC) return;
+  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(WA > WC ? WA : WC, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC);
+  NUC34_LAUNCH_CHECK();
+}
+
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, p.ctl);
   hipLaunchKernelGGL(k_peel_init, dim3(grid_for((p.T + 15) / 16, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
@@ -641,7 +717,7 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
   hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
   hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, koff, lists);
-  hipLaunchKernelGGL(k_peel_finish, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out);
+  hipLaunchKernelGGL(k_peel_finish<false>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, K4Src{});
   NUC34_LAUNCH_CHECK();
 }
 
@@ -650,7 +726,7 @@ void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, h
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
   hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
   hipLaunchKernelGGL(k_peel_debit_bits, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, src);
-  hipLaunchKernelGGL(k_peel_finish, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out);
+  hipLaunchKernelGGL(k_peel_finish<true>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, src);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -18,6 +18,12 @@ struct K4Src {
   const uint64_t* triBC;  // (b << 32) | c per triangle
   const uint32_t* cIdx;   // triangle id per c-slot
   uint64_t T;
+  // Live rows for the on-demand peel (null elsewhere): LA = (Aout|Ain) and
+  // LC = Cm bits, each triangle's three bits cleared after the round it dies
+  // in, so a dying triangle enumerates only K4s whose two partners on its
+  // lowest vertex's rows are still alive or dying in the same round.
+  uint32_t* LA;
+  uint32_t* LC;
 };
 
 // sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
@@ -86,5 +92,7 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
                               hipStream_t s);
 void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
+// LA[i] = Aout[i] | Ain[i].bits for i < WA, LC[i] = Cm[i].bits for i < WC.
+void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s);
 
 }  // namespace nuc34

@@ -74,6 +74,9 @@ __device__ __forceinline__ uint32_t atomic_min_u32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ uint32_t atomic_or_u32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint32_t atomic_and_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Wave-aggregated append: every active lane with `pred` gets a unique slot of
 // *counter; one atomic per wave.

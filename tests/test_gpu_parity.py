@@ -146,6 +146,27 @@ def test_full_size_hash(tmp_path, preset):
     assert all(np.array_equal(first, v) for v in ks.values())
 
 
+def test_full_size_rmat20k30(tmp_path):
+    """R-MAT scale 20 with 64 planted K30 (423 M triangles, 17.0 G four-cliques),
+    on-demand: the raw k and triangle arrays hash to the oracle's, and T, Q,
+    K_max and the number of peel rounds agree."""
+    exp = HASHES["rmat20k30"]
+    p = gen("preset", "rmat20k30", out=tmp_path / "rmat20k30.edges")
+    g = nuc34.load_dataset(p)
+    print("[full] rmat20k30 ondemand", flush=True)
+    with nuc34.Context() as ctx:
+        ctx.load_graph(g)
+        del g
+        st = ctx.decompose("ondemand")
+        assert (st["triangles"], st["four_cliques"], st["k_max"]) == (exp["T"], exp["Q"], exp["K_max"])
+        assert st["rounds"] == exp["rounds"] and st["unpeeled"] == 0
+        k = ctx.copy_k()
+        assert hashlib.sha256(k.tobytes()).hexdigest() == exp["k_raw_sha256"]
+        del k
+        tris = ctx.copy_triangles()
+        assert hashlib.sha256(tris.tobytes()).hexdigest() == exp["tri_raw_sha256"]
+
+
 def test_cli_matches_golden(tmp_path):
     for exe, name in (("nucleus34", "rmat8.edges"), ("nucleus34_ondemand", "rmat7k8.edges"),
                       ("nucleus34", "er40.grh"), ("nucleus34", "star.edges"), ("nucleus34_ondemand", "wheel.edges")):
