@@ -117,16 +117,25 @@ def run_traffic(workload: str, group: str, launches: float, tmp: str, calib_mib:
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(tmp, counter)
         try:
-            subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "--", *child], check=True, timeout=240,
-                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        except (subprocess.SubprocessError, OSError):
+            # the profiler can exit nonzero after writing complete results, so
+            # the database decides, not the exit status
+            rc = subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "--", *child], timeout=240,
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL).returncode
+        except (subprocess.SubprocessError, OSError) as e:
+            print(f"traffic: {counter} run failed: {e}", file=sys.stderr)
             return None
         dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("results.db")]
         if not dbs:
+            print(f"traffic: {counter} run wrote no database (exit {rc})", file=sys.stderr)
             return None
-        rows = _pmc_rows(dbs[0])
+        try:
+            rows = _pmc_rows(dbs[0])
+        except Exception as e:  # noqa: BLE001 - report and fall back to null
+            print(f"traffic: {counter} database unreadable (exit {rc}): {e}", file=sys.stderr)
+            return None
         copies = [v for n, v in rows if "copyBuffer" in n]
         if not copies:
+            print(f"traffic: {counter} database has no calibration copy (exit {rc}, {len(rows)} rows)", file=sys.stderr)
             return None
         scale = calib_mib * 1024.0 / max(copies)  # counter KiB -> true KiB
         names = GROUP_KERNELS.get(group, ())

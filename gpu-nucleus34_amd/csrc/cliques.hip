@@ -174,12 +174,13 @@ struct TileLds {
   uint32_t lpos[kWave];     // ... and list position of their first K4
 };
 
-// One tile: lane l brings triangle t (when valid). The tile's row words are
+// One tile: lane l brings triangle t (when valid). LIVE: K4 bits from the live
+// rows (the on-demand peel). The tile's row words are
 // flattened over the lanes 64 at a time, then the set bits of those 64 words
 // (the K4s) are flattened again, one K4 per lane and step, and emit(t, pos, q)
 // runs once per K4 of each triangle. With POS, pos is the K4's index in the
 // triangle's list (up words first, word and bit order); without, pos is 0.
-template <bool POS, typename Emit>
+template <bool POS, bool LIVE, typename Emit>
 __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, TileLds& L, Emit&& emit) {
   const unsigned lane = lane_id();
   uint32_t words = 0;
@@ -204,7 +205,7 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
     if (act) {
       i = find_slot(L.pre, item);
       wi = item - L.pre[i];
-      k = k4_word<false>(s, L.rows[i], wi);
+      k = k4_word<LIVE>(s, L.rows[i], wi);
     }
     const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
     const uint32_t incl = wave_inclusive_sum(pc);
@@ -236,48 +237,6 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
         uint32_t q[3];
         k4_partners(s, L.rows[ti], L.wi[sl], kw, j, q);
         emit(L.tri[ti], POS ? L.lpos[sl] + r : 0u, q);
-      }
-    }
-    wave_sync();
-  }
-}
-
-// Per-wave LDS of the single-triangle walk.
-struct OneLds {
-  K4Word word[kWave];
-  uint32_t wi[kWave];
-  uint32_t epre[kWave];
-};
-
-// One triangle on a whole wave (the on-demand peel, live rows): its row words
-// 64 at a time, then their set bits (the K4s) one per lane and step; emit(q)
-// once per K4. Every lane calls it with the same t.
-template <typename Emit>
-__device__ __forceinline__ void k4_one(const K4Src& s, uint32_t t, OneLds& L, Emit&& emit) {
-  const unsigned lane = lane_id();
-  const TriRows r = tri_rows(s, t);
-  const uint32_t words = r.W + r.WC;
-  for (uint32_t base = 0; base < words; base += kWave) {
-    const uint32_t wi = base + lane;
-    K4Word k{};
-    if (wi < words) k = k4_word<true>(s, r, wi);
-    const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
-    const uint32_t incl = wave_inclusive_sum(pc);
-    const uint32_t nk = __shfl(incl, kWave - 1);
-    if (!nk) continue;  // wave-uniform
-    L.word[lane] = k;
-    L.wi[lane] = wi;
-    L.epre[lane] = incl - pc;
-    wave_sync();
-    for (uint32_t eb = 0; eb < nk; eb += kWave) {
-      const uint32_t e = eb + lane;
-      if (e < nk) {
-        const uint32_t sl = find_slot(L.epre, e);
-        const K4Word kw = L.word[sl];
-        const uint32_t j = select_bit(kw.bits, e - L.epre[sl]);
-        uint32_t q[3];
-        k4_partners(s, r, L.wi[sl], kw, j, q);
-        emit(q);
       }
     }
     wave_sync();
@@ -320,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < s.T; tile += nw) {
     const uint64_t t = tile * kWave + lane_id();
-    k4_tile<true>(s, static_cast<uint32_t>(t < s.T ? t : 0), t < s.T, lds[w],
+    k4_tile<true, false>(s, static_cast<uint32_t>(t < s.T ? t : 0), t < s.T, lds[w],
                   [&](uint32_t tt, uint32_t pos, const uint32_t q[3]) {
                     uint32_t* o = lists + 3 * (koff[tt] + pos);
                     o[0] = q[0];
@@ -520,40 +479,53 @@ __global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p, PeelCtl* in
   }
 }
 
-// Owner test and debit for one K4 of dying triangle t with partners q[0..2].
-// A partner that is alive, or was pushed this round (dies next round), counts
-// as surviving for ownership; an earlier death means the K4 is gone; a
-// same-round death with a lower index owns it. A survivor falling from
-// level+1 to level is pushed: its k is fixed now and it dies next round.
-// kCur and kGone were set before this kernel started; a push racing with this
-// read can only be seen as kAlive, which gives the same owner, and the extra
-// debit of a pushed triangle cannot push it again (its support is already at
-// most level).
-__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
-                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
-  uint32_t sq[3];
+__device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[3], uint32_t sq[3]) {
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    sq[s] = tri_state(p.st, q[s]);
-    if (sq[s] == kGone) return;
-    if (sq[s] == kCur && q[s] < t) return;
+  for (int k = 0; k < 3; ++k) sq[k] = tri_state(p.st, q[k]);
+}
+
+// Owner test and debit for one K4 of dying triangle t with partners q[0..2]
+// in states sq[0..2]. A partner that is alive, or was pushed this round (dies
+// next round), counts as surviving for ownership; an earlier death means the
+// K4 is gone; a same-round death with a lower index owns it. A survivor
+// falling from level+1 to level is pushed: its k is fixed now and it dies next
+// round. kCur and kGone were set before this kernel started; a push racing with
+// the state read can only be seen as kAlive, which gives the same owner, and
+// the extra debit of a pushed triangle cannot push it again (its support is
+// already at most level).
+__device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3], const PeelBufs& p,
+                                         int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (sq[k] == kGone) return;
+    if (sq[k] == kCur && q[k] < t) return;
   }
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    if (sq[s] != kAlive) continue;
-    const int32_t old = atomic_sub_i32(&p.dd[q[s]], 1);
+  for (int k = 0; k < 3; ++k) {
+    if (sq[k] != kAlive) continue;
+    const int32_t old = atomic_sub_i32(&p.dd[q[k]], 1);
     if (old == level + 1) {
-      atomic_or_u32(&p.st[q[s] >> 4], kNext << ((q[s] & 15u) * 2u));
-      p.kv[q[s]] = level;
-      nxt[atomic_add_u32(ncount, 1u)] = q[s];
+      atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
+      p.kv[q[k]] = level;
+      nxt[atomic_add_u32(ncount, 1u)] = q[k];
     }
   }
 }
 
-// End of a round: advance every state word (this round's dead become kGone,
-// the pushed become the next frontier); when nothing was pushed, the min alive
-// support for the next level goes into out->minval. Workgroup 0 writes the
-// next round's control into `out`; no other workgroup reads `out`.
+__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
+                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+  uint32_t sq[3];
+  read_states(p, q, sq);
+  debit_k4(t, q, sq, p, level, nxt, ncount);
+}
+
+// Dying triangles per wave for a frontier of n: spread over every wave of the
+// grid first, then up to 64 per wave so small triangles share a wave.
+__device__ __forceinline__ uint32_t tile_width(uint32_t n, uint32_t waves) {
+  const uint32_t g = n / waves;
+  return g < 1 ? 1u : (g > static_cast<uint32_t>(kWave) ? static_cast<uint32_t>(kWave) : g);
+}
+
 // Clears the live-row bits of this round's dying triangles (on-demand only;
 // after the debit, so same-round ownership still sees every K4).
 __device__ __forceinline__ void unlink_frontier(const PeelBufs& p, const PeelCtl* in, const K4Src& s) {
@@ -617,6 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
   }
 }
 
+// Direct debit: one wave per dying triangle, one K4 list entry per lane.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in,
                                                              const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
@@ -641,10 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   }
 }
 
-// On-demand debit: one wave per dying triangle, its K4s enumerated from the
-// bit rows (rounds with few dying triangles still spread over many waves).
+// On-demand debit: a wave takes tile_width() dying triangles; their K4s are
+// enumerated from the live bit rows, flattened over the lanes.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, K4Src s) {
-  __shared__ OneLds lds[kWPB];
+  __shared__ TileLds lds[kWPB];
   if (in->done) return;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
@@ -652,10 +625,14 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   uint32_t* ncount = &in->nnext;
   const uint32_t n = in->ncur;
   const unsigned w = threadIdx.x / kWave;
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
-    const uint32_t t = cur[i];
-    k4_one(s, t, lds[w], [&](const uint32_t q[3]) { destroy_k4(t, q, p, level, nxt, ncount); });
+  const unsigned lane = lane_id();
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  const uint32_t G = tile_width(n, static_cast<uint32_t>(nw));
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
+    const uint64_t i = tile * G + lane;
+    const bool valid = lane < G && i < n;
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w],
+                         [&](uint32_t tt, uint32_t, const uint32_t q[3]) { destroy_k4(tt, q, p, level, nxt, ncount); });
   }
 }
 

@@ -102,6 +102,17 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
   return v;
 }
 
+// Wave-wide inclusive prefix sum of a 64-bit value.
+__device__ __forceinline__ uint64_t wave_inclusive_sum_u64(uint64_t v) {
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint64_t t = __shfl_up(v, static_cast<unsigned>(off));
+    if (lane >= static_cast<unsigned>(off)) v += t;
+  }
+  return v;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

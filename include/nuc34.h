@@ -150,8 +150,10 @@ int nuc34_copy_triangles(nuc34_ctx* ctx, uint64_t first, uint64_t count, uint32_
 int nuc34_support(nuc34_ctx* ctx, uint64_t* Q);
 int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
 
-/* Runs the missing stages and the peel. variant: NUC34_DIRECT, NUC34_ONDEMAND or
- * NUC34_AUTO (direct when its K4 list fits the budget). */
+/* Runs the missing stages and the peel. variant: NUC34_DIRECT (per-triangle
+ * K4 lists in HBM, 48 bytes per K4), NUC34_ONDEMAND (K4s re-enumerated from the
+ * bit-packed incidence as triangles die, O(T) memory) or NUC34_AUTO (on-demand:
+ * never slower and never larger on the measured workloads). */
 int nuc34_decompose(nuc34_ctx* ctx, int variant, uint32_t parts, nuc34_stats* stats);
 /* k for every triangle in (a,b) order (0 for triangles in no K4). */
 int nuc34_copy_k(nuc34_ctx* ctx, int32_t* k);
