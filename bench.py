@@ -103,6 +103,22 @@ def _pmc_rows(db_path: str):
     return [(n.replace("(anonymous namespace)::", "").split("(")[0], v) for n, v in rows]
 
 
+def calibration_copy(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the HIP runtime (no torch:
+    with torch loaded, the profiler's counter mode breaks HIP event calls)."""
+    import ctypes as C
+    path = "/opt/rocm/lib/libamdhip64.so"
+    hip = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for call in (lambda: hip.hipMalloc(C.byref(a), n), lambda: hip.hipMalloc(C.byref(b), n),
+                 lambda: hip.hipMemset(a, 1, n), lambda: hip.hipMemcpy(b, a, n, 3),  # 3: device to device
+                 lambda: hip.hipDeviceSynchronize(), lambda: hip.hipFree(a), lambda: hip.hipFree(b)):
+        rc = call()
+        if rc != 0:
+            raise RuntimeError(f"HIP error {rc} in the calibration copy")
+
+
 def run_traffic(workload: str, group: str, launches: float, tmp: str, calib_mib: int = 1024) -> dict | None:
     """HBM bytes per launch of `group` from two rocprofv3 --pmc runs of a
     one-step bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
@@ -222,11 +238,7 @@ def main():
     import nuc34
 
     if args.copy_calibration_mib:
-        import torch
-        a = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
-        torch.empty_like(a).copy_(a)
-        torch.cuda.synchronize()
-        del a
+        calibration_copy(args.copy_calibration_mib)
     wl = WORKLOADS[args.workload]
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "graph.edges")

@@ -88,8 +88,11 @@ int guard(nuc34_ctx* c, F&& f) {
     return fail(c, NUC34_E_HIP, e.what());
   } catch (const std::invalid_argument& e) {
     return fail(c, NUC34_E_INVALID, e.what());
-  } catch (const std::exception& e) {
+  } catch (const std::logic_error& e) {
     return fail(c, NUC34_E_INVALID, e.what());
+  } catch (const std::exception& e) {
+    // thrown from inside the runtime: name the call
+    return fail(c, NUC34_E_HIP, std::string(e.what()) + " (in " + tls_hip_call + ")");
   }
 }
 
@@ -409,9 +412,9 @@ void run_peel(nuc34_ctx* c, int variant) {
       });
       NUC34_HIP(hipMemcpyAsync(hs, ctlb.as<void>(), sizeof(hs), hipMemcpyDeviceToHost, c->s));
       NUC34_HIP(hipStreamSynchronize(c->s));
-      // the round that finished writes done into both slots; its successor's
-      // slot (the higher round) holds the final counts
-      h = hs[0].round >= hs[1].round ? hs[0] : hs[1];
+      // the last round written (the higher sequence number) holds the final
+      // counts; the round that finishes marks both slots done
+      h = hs[0].seq >= hs[1].seq ? hs[0] : hs[1];
       if (h.done) break;
       if (l > max_launches) throw std::logic_error("peel did not finish");
     }

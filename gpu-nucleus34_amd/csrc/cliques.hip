@@ -341,13 +341,6 @@ __device__ __forceinline__ uint32_t tri_state(const uint32_t* st, uint32_t t) {
   return (st[t >> 4] >> ((t & 15u) * 2u)) & 3u;
 }
 
-// Start of a round for 16 fields at once: kCur -> kGone, kNext -> kCur, kAlive
-// and kGone unchanged (new high bit = old low bit, new low bit = low | high).
-__device__ __forceinline__ uint32_t advance_states(uint32_t w) {
-  const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
-  return (lo << 1) | lo | hi;
-}
-
 // Low bit of every kAlive field of w.
 __device__ __forceinline__ uint32_t alive_fields(uint32_t w) { return ~(w | (w >> 1)) & 0x55555555u; }
 
@@ -389,11 +382,12 @@ __device__ __forceinline__ void block_min_to(uint32_t mn, uint32_t* out) {
 __global__ void k_peel_ctl_init(PeelCtl* ctl) {
   for (int s = 0; s < 2; ++s) {
     PeelCtl& c = ctl[s];
-    c.ncur = c.nnext = c.sel = 0;
-    c.need_level = s == 0 ? 1u : 0u;
+    c.ncur = c.nspec = c.nnext = c.sel = 0;
+    c.need_level = s == 0 ? 1u : 0u;  // round 0 collects the first level
     c.minval = kNoMin;
     c.level = 0;
     c.round = 0;
+    c.seq = static_cast<uint32_t>(s);
     c.done = 0;
     c.error = 0;
     c.peeled = 0;
@@ -424,59 +418,6 @@ __global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t*
     p.st[i] = w;
   }
   block_min_to(mn, &p.ctl[0].minval);
-}
-
-// Level change (the frontier ran empty): every alive triangle with support
-// <= level dies this round (k = level). One state word (16 triangles) per
-// thread; one atomic per workgroup tile reserves the frontier slots.
-__global__ __launch_bounds__(kBlock) void k_peel_collect(PeelBufs p, PeelCtl* in) {
-  if (in->done || !in->need_level) return;
-  __shared__ uint32_t wd[kWPB], base;
-  const int32_t level = round_level(in);
-  uint32_t* cur = in->sel ? p.buf1 : p.buf0;
-  const uint64_t W = (p.T + 15) / 16;
-  const unsigned wv = threadIdx.x / kWave;
-  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kBlock; b0 < W; b0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
-    const uint64_t i = b0 + threadIdx.x;
-    uint32_t w = 0, die = 0;
-    if (i < W) {
-      w = p.st[i];
-      const uint32_t al = alive_fields(w);
-      if (al) {
-        int32_t d[16];
-        load_word_dd(p.dd, i, d);
-#pragma unroll
-        for (int f = 0; f < 16; ++f)
-          if (((al >> (2 * f)) & 1u) && d[f] <= level) die |= 1u << f;
-      }
-    }
-    const uint32_t nd = static_cast<uint32_t>(__popc(die));
-    const uint32_t id = wave_inclusive_sum(nd);
-    if (lane_id() == kWave - 1) wd[wv] = id;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t td = 0;
-      for (int q = 0; q < kWPB; ++q) {
-        const uint32_t x = wd[q];
-        wd[q] = td;
-        td += x;
-      }
-      base = td ? atomic_add_u32(&in->ncur, td) : 0;
-    }
-    __syncthreads();
-    if (die) {
-      uint32_t od = base + wd[wv] + id - nd;
-      for (int f = 0; f < 16; ++f) {
-        if (!((die >> f) & 1u)) continue;
-        const uint32_t t = static_cast<uint32_t>(16 * i + f);
-        p.kv[t] = level;
-        cur[od++] = t;
-        w |= kCur << (2 * f);
-      }
-      p.st[i] = w;
-    }
-    __syncthreads();
-  }
 }
 
 __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[3], uint32_t sq[3]) {
@@ -526,61 +467,103 @@ __device__ __forceinline__ uint32_t tile_width(uint32_t n, uint32_t waves) {
   return g < 1 ? 1u : (g > static_cast<uint32_t>(kWave) ? static_cast<uint32_t>(kWave) : g);
 }
 
-// Clears the live-row bits of this round's dying triangles (on-demand only;
-// after the debit, so same-round ownership still sees every K4).
-__device__ __forceinline__ void unlink_frontier(const PeelBufs& p, const PeelCtl* in, const K4Src& s) {
-  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
-  const uint32_t n = in->ncur;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = cur[i];
-    const uint32_t a = s.triA[t];
-    const uint64_t bc = s.triBC[t];
-    const uint32_t b = static_cast<uint32_t>(bc >> 32), c = static_cast<uint32_t>(bc);
-    const uint32_t ou = s.O[s.src[b]];
-    const uint32_t jv = b - ou, jw = a - ou, ip = s.inpos[b];
-    atomic_and_u32(&s.LA[s.rows.rowA[a] + (jv >> 5)], ~(1u << (jv & 31)));  // t in a's row
-    atomic_and_u32(&s.LA[s.rows.rowA[b] + (jw >> 5)], ~(1u << (jw & 31)));  // t in b's row
-    atomic_and_u32(&s.LC[s.rows.rowC[c] + (ip >> 5)], ~(1u << (ip & 31)));  // t in c's c-role row
-  }
-}
-
 template <bool UNLINK>
 __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
+  __shared__ uint32_t wd[kWPB], base;
   if (in->done) return;
-  if (UNLINK) unlink_frontier(p, in, s);
-  const bool need = in->nnext == 0;
-  const uint64_t W = (p.T + 15) / 16;
-  uint32_t mn = kNoMin;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < W;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t w = advance_states(p.st[i]);
-    p.st[i] = w;
-    if (need) {
-      const uint32_t al = alive_fields(w);
-      if (al) {
-        int32_t d[16];
-        load_word_dd(p.dd, i, d);
-#pragma unroll
-        for (int f = 0; f < 16; ++f)
-          if ((al >> (2 * f)) & 1u) mn = min(mn, static_cast<uint32_t>(d[f] < 0 ? 0 : d[f]));
-      }
+  const uint32_t F = in->ncur + in->nspec;
+  const bool collect_round = F == 0;
+  const bool need = !collect_round && in->nnext == 0;
+  const int32_t L = round_level(in);
+  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t gsize = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = gtid; i < F; i += gsize) {
+    const uint32_t t = cur[i];
+    atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
+    if (UNLINK) {
+      const uint32_t a = s.triA[t];
+      const uint64_t bc = s.triBC[t];
+      const uint32_t b = static_cast<uint32_t>(bc >> 32), c = static_cast<uint32_t>(bc);
+      const uint32_t ou = s.O[s.src[b]];
+      const uint32_t jv = b - ou, jw = a - ou, ip = s.inpos[b];
+      atomic_and_u32(&s.LA[s.rows.rowA[a] + (jv >> 5)], ~(1u << (jv & 31)));  // t in a's row
+      atomic_and_u32(&s.LA[s.rows.rowA[b] + (jw >> 5)], ~(1u << (jw & 31)));  // t in b's row
+      atomic_and_u32(&s.LC[s.rows.rowC[c] + (ip >> 5)], ~(1u << (ip & 31)));  // t in c's c-role row
     }
   }
-  if (need) block_min_to(mn, &out->minval);
+  const uint32_t nn = in->nnext;
+  for (uint64_t i = gtid; i < nn; i += gsize) {
+    const uint32_t t = nxt[i];
+    atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
+  }
+  if (need || collect_round) {
+    const int32_t take = collect_round ? L : L + 1;  // also the k of the collected triangles
+    const uint64_t W = (p.T + 15) / 16;
+    const unsigned wv = threadIdx.x / kWave;
+    uint32_t mn = kNoMin;
+    for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kBlock; b0 < W; b0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
+      const uint64_t i = b0 + threadIdx.x;
+      uint32_t die = 0;
+      if (i < W) {
+        const uint32_t al = alive_fields(p.st[i]);
+        if (al) {
+          int32_t d[16];
+          load_word_dd(p.dd, i, d);
+#pragma unroll
+          for (int f = 0; f < 16; ++f) {
+            if (!((al >> (2 * f)) & 1u)) continue;
+            mn = min(mn, static_cast<uint32_t>(d[f] < 0 ? 0 : d[f]));
+            if (d[f] <= take) die |= 1u << f;
+          }
+        }
+      }
+      const uint32_t nd = static_cast<uint32_t>(__popc(die));
+      const uint32_t id = wave_inclusive_sum(nd);
+      if (lane_id() == kWave - 1) wd[wv] = id;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t td = 0;
+        for (int q = 0; q < kWPB; ++q) {
+          const uint32_t x = wd[q];
+          wd[q] = td;
+          td += x;
+        }
+        base = td ? atomic_add_u32(&out->nspec, td) : 0;
+      }
+      __syncthreads();
+      if (die) {
+        uint32_t od = base + wd[wv] + id - nd;
+        uint32_t set = 0;
+        for (int f = 0; f < 16; ++f) {
+          if (!((die >> f) & 1u)) continue;
+          const uint32_t t = static_cast<uint32_t>(16 * i + f);
+          p.kv[t] = take;
+          nxt[od++] = t;
+          set |= kCur << (2 * f);
+        }
+        atomic_or_u32(&p.st[i], set);  // 00 -> 01
+      }
+      __syncthreads();
+    }
+    if (need) block_min_to(mn, &out->minval);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const uint64_t peeled = in->peeled + in->ncur;
+    const uint64_t peeled = in->peeled + F;
     uint32_t error = 0, done = 0;
-    if (in->need_level && (in->minval == kNoMin || in->ncur == 0)) {
-      error = in->minval == kNoMin ? 1u : 2u;  // nothing alive / nothing collected
+    if (collect_round && (!in->need_level || in->minval == kNoMin)) {
+      error = in->minval == kNoMin ? 1u : 2u;  // nothing alive / collection found nothing
       done = 1;
     }
     if (peeled >= p.T) done = 1;
-    out->ncur = in->nnext;
+    out->ncur = nn;
     out->nnext = 0;
     out->sel = in->sel ^ 1u;
     out->need_level = need ? 1u : 0u;
-    out->level = round_level(in);
-    out->round = in->round + 1;
+    out->level = L;
+    out->round = in->round + (F ? 1u : 0u);
+    out->seq = in->seq + 1;
     out->error = error;
     out->peeled = peeled;
     out->done = done;
@@ -590,15 +573,16 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
 }
 
 // Direct debit: one wave per dying triangle, one K4 list entry per lane.
-__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in,
+__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
                                                              const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
   if (in->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;  // this round's finish counts into it
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   uint32_t* ncount = &in->nnext;
-  const uint32_t n = in->ncur;
+  const uint32_t n = in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint32_t nw = gridDim.x * kWPB;
@@ -616,14 +600,15 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
 // enumerated from the live bit rows, flattened over the lanes.
-__global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, K4Src s) {
+__global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
   __shared__ TileLds lds[kWPB];
   if (in->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;  // this round's finish counts into it
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   uint32_t* ncount = &in->nnext;
-  const uint32_t n = in->ncur;
+  const uint32_t n = in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
@@ -692,8 +677,7 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
                               hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
-  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
-  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, koff, lists);
+  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, out, koff, lists);
   hipLaunchKernelGGL(k_peel_finish<false>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, K4Src{});
   NUC34_LAUNCH_CHECK();
 }
@@ -701,8 +685,7 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
 void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
-  hipLaunchKernelGGL(k_peel_collect, dim3(kScanGrid), dim3(kBlock), 0, s, p, in);
-  hipLaunchKernelGGL(k_peel_debit_bits, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, src);
+  hipLaunchKernelGGL(k_peel_debit_bits, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, out, src);
   hipLaunchKernelGGL(k_peel_finish<true>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, src);
   NUC34_LAUNCH_CHECK();
 }

@@ -60,16 +60,17 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
 // that another workgroup of the same kernel reads. Rounds are enqueued in
 // groups of even size, so a round's parity is its slot parity in the group.
 struct PeelCtl {
-  uint32_t ncur;        // size of this round's frontier
-  uint32_t nnext;       // size of the next round's frontier (the debit appends)
+  uint32_t ncur;        // this round's frontier: pushed by the previous round ...
+  uint32_t nspec;       // ... or collected by the previous round's finish
+  uint32_t nnext;       // next round's frontier (the debit appends)
   uint32_t sel;         // buffer holding this round's frontier
-  uint32_t need_level;  // frontier empty: this round picks the next level
-  uint32_t minval;      // min support over alive triangles (when need_level)
+  uint32_t need_level;  // the frontier came from a level change: level = max(level, minval)
+  uint32_t minval;      // min support over alive triangles at that level change
   int32_t level;
-  uint32_t round;
+  uint32_t round;       // rounds with a non-empty frontier so far
+  uint32_t seq;         // rounds of any kind so far (the later slot has the larger value)
   uint32_t done;
   uint32_t error;       // nonzero: inconsistent state detected
-  uint32_t pad;
   uint64_t peeled;
 };
 
@@ -83,11 +84,12 @@ struct PeelBufs {
   uint64_t T;
 };
 
-// One peel round at position `slot` of its group: level selection when the
-// frontier is empty, the debit of every K4 owned by a dying triangle, and the
-// state advance. Every kernel returns at once when the round's control says
-// done, so extra rounds are no-ops. The direct round reads the K4 lists; the
-// on-demand round enumerates each dying triangle's K4s from the bit rows.
+// One peel round at position `slot` of its group: the debit of every K4 owned
+// by a dying triangle, then the state advance and, when the frontier ran
+// empty, the next level's collection. Both kernels return at once when the
+// round's control says done, so extra rounds are no-ops. The direct round
+// reads the K4 lists; the on-demand round enumerates each dying triangle's K4s
+// from the bit rows.
 void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
                               hipStream_t s);
 void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s);

@@ -27,8 +27,13 @@ struct OutOfMemory : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// The HIP call in progress on this thread, named in error messages when the
+// runtime (or a tool attached to it) throws instead of returning an error.
+inline thread_local const char* tls_hip_call = "";
+
 #define NUC34_HIP(expr)                                                                   \
   do {                                                                                    \
+    ::nuc34::tls_hip_call = #expr;                                                        \
     hipError_t e_ = (expr);                                                               \
     if (e_ != hipSuccess) throw ::nuc34::HipError(e_, std::string(#expr) + " at " __FILE__); \
   } while (0)
@@ -76,6 +81,9 @@ __device__ __forceinline__ uint32_t atomic_or_u32(uint32_t* p, uint32_t v) {
 }
 __device__ __forceinline__ uint32_t atomic_and_u32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_xor_u32(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wave-aggregated append: every active lane with `pred` gets a unique slot of
