@@ -572,12 +572,19 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
   }
 }
 
+// Zeroes the count the finish of this round collects into. Done after the
+// kernel's last read of its own control: a store through `out` earlier (it
+// may alias `in` as far as the compiler knows) turns every later read of the
+// round's control into a vector load.
+__device__ __forceinline__ void reset_next_count(PeelCtl* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;
+}
+
 // Direct debit: one wave per dying triangle, one K4 list entry per lane.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
                                                              const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
   if (in->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;  // this round's finish counts into it
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
@@ -596,6 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
       destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
+  reset_next_count(out);
 }
 
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
@@ -603,7 +611,6 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
   __shared__ TileLds lds[kWPB];
   if (in->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;  // this round's finish counts into it
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
@@ -619,6 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
     k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w],
                          [&](uint32_t tt, uint32_t, const uint32_t q[3]) { destroy_k4(tt, q, p, level, nxt, ncount); });
   }
+  reset_next_count(out);
 }
 
 __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
