@@ -217,6 +217,73 @@ def run_batched(nuc34, count: int, seed: int = 3) -> dict:
             "graphs_per_s_plain": count / plain_s, "speedup": plain_s / (t1 - t0), "bit_exact_vs_plain": bool(exact)}
 
 
+def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: int = 4, pool: int = 256) -> dict:
+    """Serving workload: Poisson arrivals at `rate` jobs/s for `seconds`, seed
+    `seed`; each job is a graph of the seeded mixed batch (pool of `pool`
+    graphs) or, with probability `large_share`, an R-MAT scale 18 edge factor 16
+    graph (two seeds). Jobs go to a nuc34 server in real time; a collector
+    thread takes every result and checks it bit-exact against the same graph's
+    own decomposition. Latency = result on the host minus submission."""
+    import queue
+    import threading
+
+    rng = np.random.default_rng(seed)
+    small = [nuc34.gen_batch_graph(seed, i) for i in range(pool)]
+    large = [nuc34.gen_rmat(18, 16, 0.57, 0.19, 0.19, s_, 0, 0) for s_ in (1, 2)]
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        big = bool(rng.random() < large_share)
+        arrivals.append((t, big, int(rng.integers(len(large) if big else len(small)))))
+    used = sorted({(b, i) for _, b, i in arrivals})
+    expect = {}
+    with nuc34.Context(0) as ctx:  # reference results: each graph decomposed on its own
+        for b, i in used:
+            ctx.load_graph(large[i] if b else small[i])
+            ctx.decompose("auto")
+            expect[(b, i)] = (ctx.copy_triangles(), ctx.copy_k())
+    results, bad = [], []
+    q = queue.Queue()
+    with nuc34.Server(0) as srv:
+        def collect():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                job, key = item
+                info = srv.wait(job, -1)
+                tris, k = srv.take(job, info["triangles"])
+                et, ek = expect[key]
+                if info["status"] != nuc34.OK or not (np.array_equal(k, ek) and np.array_equal(tris, et)):
+                    bad.append((key, info["status"], info["error"]))
+                results.append((key[0], info["t_submit"], info["t_done"], info["batch_graphs"]))
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = srv.now()
+        for ta, b, i in arrivals:
+            d = t0 + ta - srv.now()
+            if d > 0:
+                time.sleep(d)
+            q.put((srv.submit(large[i] if b else small[i]), (b, i)))
+        q.put(None)
+        th.join()
+        st = srv.stats()
+    lat = {c: np.array([r[2] - r[1] for r in results if r[0] == c]) for c in (False, True)}
+    allat = np.array([r[2] - r[1] for r in results])
+    span = max(r[2] for r in results) - min(r[1] for r in results)
+    pct = lambda a, p: float(np.percentile(a, p)) * 1e3 if len(a) else None
+    return {"seconds": seconds, "rate": rate, "large_share": large_share, "seed": seed, "jobs": len(results),
+            "large_jobs": int(sum(r[0] for r in results)), "bit_exact": not bad, "mismatches": bad[:5],
+            "graphs_per_s": len(results) / span, "span_s": span,
+            "p50_ms": pct(allat, 50), "p99_ms": pct(allat, 99),
+            "small_p50_ms": pct(lat[False], 50), "small_p99_ms": pct(lat[False], 99),
+            "large_p50_ms": pct(lat[True], 50), "large_p99_ms": pct(lat[True], 99),
+            "mean_batch_graphs": float(np.mean([r[3] for r in results if not r[0]])) if lat[False].size else None,
+            "server": st}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +295,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain comparison")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="also run the serving workload (Poisson arrivals) for this many seconds of arrivals")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-large-share", type=float, default=0.05)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first make a device-to-device copy of this many MiB (counter calibration under --pmc)")
     args = ap.parse_args()
@@ -259,6 +330,8 @@ def main():
         t1 = time.perf_counter()
         ctx.close()
         batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
+        serving = (run_serving(nuc34, args.serve_seconds, args.serve_rate, args.serve_large_share)
+                   if args.serve_seconds > 0 else None)
         cpu = None if args.no_cpu_baseline else run_cpu_baseline(tmp, args.cpu_threads)
         st = steps[-1]
         kern = {}
@@ -306,6 +379,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms},
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(rec))
 

@@ -652,6 +652,13 @@ int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t 
 int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
                      const uint32_t* E, int oriented) {
   if (!c || !n || !m || !O || (count && !E)) return fail(c, NUC34_E_INVALID, "null argument");
+  if (count == 1) {  // the union of one graph is the graph
+    const int rc = nuc34_load_csr(c, O, E, n[0], m[0], oriented);
+    if (rc != NUC34_OK) return rc;
+    c->batch_v = {0, n[0]};
+    c->batch_e = {0, c->m};
+    return NUC34_OK;
+  }
   std::vector<uint32_t> Ou, Eu;
   std::vector<uint64_t> vfirst(count + 1, 0), efirst_in(count + 1, 0);
   for (uint32_t g = 0; g < count; ++g) {
@@ -757,18 +764,17 @@ int nuc34_copy_triangles(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t*
   if (!c || !c->has_tris) return fail(c, NUC34_E_STATE, "triangles not computed");
   if (first > c->T || count > c->T - first) return fail(c, NUC34_E_INVALID, "triangle range out of bounds");
   return guard(c, [&]() -> int {
-    std::vector<uint32_t> a(count);
-    std::vector<uint64_t> bc(count);
-    if (count) {
-      NUC34_HIP(hipMemcpyAsync(a.data(), c->triA.as<uint32_t>() + first, count * 4, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipMemcpyAsync(bc.data(), c->triBC.as<uint64_t>() + first, count * 8, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
+    if (!count) return NUC34_OK;
+    // formatted as (a, b, c) triples on the device, copied back in chunks
+    const uint64_t chunk = std::min<uint64_t>(count, 16ull << 20);
+    DevBuf buf = c->alloc.get(chunk * 12, "triangle copy");
+    for (uint64_t done = 0; done < count; done += chunk) {
+      const uint64_t n = std::min(chunk, count - done);
+      launch_tri_abc(c->triA.as<uint32_t>() + first + done, c->triBC.as<uint64_t>() + first + done, n,
+                     buf.as<uint32_t>(), c->s);
+      NUC34_HIP(hipMemcpyAsync(abc + 3 * done, buf.as<void>(), n * 12, hipMemcpyDeviceToHost, c->s));
     }
-    for (uint64_t i = 0; i < count; ++i) {
-      abc[3 * i] = a[i];
-      abc[3 * i + 1] = static_cast<uint32_t>(bc[i] >> 32);
-      abc[3 * i + 2] = static_cast<uint32_t>(bc[i]);
-    }
+    NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;
   });
 }

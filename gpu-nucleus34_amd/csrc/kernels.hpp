@@ -21,6 +21,8 @@ void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, 
                      const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
                      hipStream_t s);
 
+// abc[3i..3i+2] = (a, b, c) of triangles [0, count) of the given arrays.
+void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s);
 // out[i] = src[idx[i]] for i < n (idx in bounds of src, checked by the caller).
 void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s);
 void launch_iota(uint32_t* a, uint64_t n, hipStream_t s);

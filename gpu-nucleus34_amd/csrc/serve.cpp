@@ -1,14 +1,16 @@
-// Serving loop over the C ABI: a queue of independent graphs, two worker
-// lanes (small jobs batched as disjoint unions, large jobs one at a time),
-// each lane with its own device context and stream, so a large job never
-// holds small ones back and the two share the device concurrently.
+// Serving loop over the C ABI: a queue of independent graphs per job class
+// (small jobs, batched as disjoint unions; large jobs, one at a time) and
+// worker threads with their own device contexts and streams (one for small
+// jobs, `large_workers` for large ones), so a large job never holds small ones
+// back, and one large job's host work overlaps another's device work.
 //
 // The reference runs one graph per process (nucleus/cuNucleus34.cu:147-175);
 // its memory plans (nucleus/utils.cuh:1329-1340, nucleus/cuNucleus34.cu:563-585)
-// decide between one-shot and segmented work from free memory. Here the same
-// decision is per batch: the support count gives Q, the context picks direct
-// when the K4 lists fit its budget, and a batch that still exceeds the budget
-// is split in halves and retried, down to single jobs.
+// decide between one-shot and segmented work from free memory. Here each lane
+// holds a share of the HBM budget; a batch runs in the configured variant (by
+// default on demand, O(T) memory, or direct with 48 bytes per K4 once the
+// support count has given Q), and a batch that exceeds the lane's budget is
+// split in halves and retried, down to single jobs.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -39,11 +41,11 @@ struct Job {
   bool done = false;
 };
 
-struct Lane {
+// A worker thread with its own context (and stream) serving one job class.
+struct Worker {
   nuc34_ctx* ctx = nullptr;
-  std::deque<Job*> queue;
-  std::thread worker;
-  bool batched = true;
+  int cls = 0;  // 0: small jobs, batched; 1: large jobs, one at a time
+  std::thread thread;
 };
 
 void set_error(nuc34_job_info& info, const char* msg) {
@@ -62,19 +64,20 @@ struct nuc34_server {
   uint64_t next_id = 1;
   uint64_t next_batch = 1;
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
-  Lane lanes[2];  // 0: small jobs, batched; 1: large jobs, one at a time
+  std::deque<Job*> queue[2];  // per class: 0 small, 1 large
+  std::vector<Worker> workers;
   nuc34_serve_stats st{};
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
-  void lane_loop(int li);
-  void run_batch(int li, std::vector<Job*>& batch);
-  int decompose_into(int li, std::vector<Job*>& batch, uint64_t batch_id, double t_start);
+  void work(Worker* w);
+  void run_batch(Worker* w, std::vector<Job*>& batch);
+  int decompose_into(Worker* w, std::vector<Job*>& batch, uint64_t batch_id, double t_start);
 };
 
 // One attempt at a batch: union load, decomposition, results sliced per job.
 // Returns the status; on NUC34_E_OOM the caller splits the batch.
-int nuc34_server::decompose_into(int li, std::vector<Job*>& batch, uint64_t batch_id, double t_start) {
-  nuc34_ctx* ctx = lanes[li].ctx;
+int nuc34_server::decompose_into(Worker* w, std::vector<Job*>& batch, uint64_t batch_id, double t_start) {
+  nuc34_ctx* ctx = w->ctx;
   const uint32_t count = static_cast<uint32_t>(batch.size());
   std::vector<uint32_t> ns(count), Ocat, Ecat;
   std::vector<uint64_t> ms(count);
@@ -85,16 +88,41 @@ int nuc34_server::decompose_into(int li, std::vector<Job*>& batch, uint64_t batc
     tv += batch[g]->n + 1;
     te += batch[g]->m;
   }
-  Ocat.reserve(tv);
-  Ecat.reserve(te);
-  for (Job* j : batch) {
-    Ocat.insert(Ocat.end(), j->O.begin(), j->O.end());
-    Ecat.insert(Ecat.end(), j->E.begin(), j->E.end());
+  const uint32_t *O = batch[0]->O.data(), *E = batch[0]->E.data();
+  if (count > 1) {
+    Ocat.reserve(tv);
+    Ecat.reserve(te);
+    for (Job* j : batch) {
+      Ocat.insert(Ocat.end(), j->O.begin(), j->O.end());
+      Ecat.insert(Ecat.end(), j->E.begin(), j->E.end());
+    }
+    O = Ocat.data();
+    E = Ecat.data();
   }
   // with no edges at all E is never read; any non-null pointer will do
-  int rc = nuc34_load_batch(ctx, count, ns.data(), ms.data(), Ocat.data(), Ecat.empty() ? Ocat.data() : Ecat.data(), 0);
+  int rc = nuc34_load_batch(ctx, count, ns.data(), ms.data(), O, te ? E : O, 0);
   nuc34_stats s{};
   if (rc == NUC34_OK) rc = nuc34_decompose(ctx, cfg.variant, 1, &s);
+  if (rc == NUC34_OK && count == 1) {  // one job: results straight into it
+    Job* j = batch[0];
+    j->abc.resize(3 * s.triangles);
+    j->k.resize(s.triangles);
+    rc = nuc34_copy_triangles(ctx, 0, s.triangles, j->abc.data());
+    if (rc == NUC34_OK) rc = nuc34_copy_k(ctx, j->k.data());
+    if (rc == NUC34_OK) {
+      j->info.status = NUC34_OK;
+      j->info.error[0] = 0;
+      j->info.variant = s.variant;
+      j->info.m = s.m;
+      j->info.triangles = s.triangles;
+      j->info.four_cliques = s.four_cliques;
+      j->info.k_max = s.triangles ? s.k_max : 0;
+      j->info.batch_graphs = 1;
+      j->info.batch_id = batch_id;
+      j->info.t_start = t_start;
+      return NUC34_OK;
+    }
+  }
   std::vector<uint64_t> ef(count + 1), tf(count + 1);
   std::vector<uint32_t> abc;
   std::vector<int32_t> k, sup;
@@ -142,7 +170,7 @@ int nuc34_server::decompose_into(int li, std::vector<Job*>& batch, uint64_t batc
   return NUC34_OK;
 }
 
-void nuc34_server::run_batch(int li, std::vector<Job*>& batch) {
+void nuc34_server::run_batch(Worker* w, std::vector<Job*>& batch) {
   uint64_t bid;
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -150,7 +178,7 @@ void nuc34_server::run_batch(int li, std::vector<Job*>& batch) {
     st.batches++;
   }
   const double t_start = now();
-  const int rc = decompose_into(li, batch, bid, t_start);
+  const int rc = decompose_into(w, batch, bid, t_start);
   if (rc == NUC34_E_OOM && batch.size() > 1) {
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -158,14 +186,14 @@ void nuc34_server::run_batch(int li, std::vector<Job*>& batch) {
     }
     std::vector<Job*> lo(batch.begin(), batch.begin() + static_cast<std::ptrdiff_t>(batch.size() / 2));
     std::vector<Job*> hi(batch.begin() + static_cast<std::ptrdiff_t>(batch.size() / 2), batch.end());
-    run_batch(li, lo);
-    run_batch(li, hi);
+    run_batch(w, lo);
+    run_batch(w, hi);
     return;
   }
   const double t_done = now();
   {
     std::lock_guard<std::mutex> lk(mu);
-    st.busy_s[li] += t_done - t_start;
+    st.busy_s[w->cls] += t_done - t_start;
     for (Job* j : batch) {
       j->info.t_done = t_done;
       if (j->info.status == NUC34_OK) st.completed++;
@@ -180,31 +208,33 @@ void nuc34_server::run_batch(int li, std::vector<Job*>& batch) {
   cv_done.notify_all();
 }
 
-void nuc34_server::lane_loop(int li) {
-  Lane& L = lanes[li];
+void nuc34_server::work(Worker* w) {
+  std::deque<Job*>& Q = queue[w->cls];
+  const bool batched = w->cls == 0;
   const uint64_t max_edges = cfg.max_batch_edges;
   const uint32_t max_graphs = cfg.max_batch_graphs;
   for (;;) {
     std::vector<Job*> batch;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stop || !L.queue.empty(); });
-      if (L.queue.empty()) return;  // stop requested and drained
-      if (L.batched && cfg.batch_window_ms > 0) {
-        const double until = L.queue.front()->info.t_submit + cfg.batch_window_ms * 1e-3;
+      cv_work.wait(lk, [&] { return stop || !Q.empty(); });
+      if (Q.empty()) return;  // stop requested and drained
+      if (batched && cfg.batch_window_ms > 0) {
+        const double until = Q.front()->info.t_submit + cfg.batch_window_ms * 1e-3;
         const auto deadline = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(until));
         cv_work.wait_until(lk, deadline, [&] { return stop; });
+        if (Q.empty()) continue;  // another worker of the class took them
       }
       uint64_t edges = 0;
-      while (!L.queue.empty()) {
-        Job* j = L.queue.front();
-        if (!batch.empty() && (!L.batched || batch.size() >= max_graphs || edges + j->m > max_edges)) break;
+      while (!Q.empty()) {
+        Job* j = Q.front();
+        if (!batch.empty() && (!batched || batch.size() >= max_graphs || edges + j->m > max_edges)) break;
         batch.push_back(j);
         edges += j->m;
-        L.queue.pop_front();
+        Q.pop_front();
       }
     }
-    run_batch(li, batch);
+    run_batch(w, batch);
   }
 }
 
@@ -221,6 +251,7 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
   if (!c.large_edges) c.large_edges = 4ull << 20;
   if (!c.max_batch_edges) c.max_batch_edges = 256ull << 20;
   if (!c.max_batch_graphs) c.max_batch_graphs = 4096;
+  if (!c.large_workers) c.large_workers = 2;
   if (!c.hbm_budget) {
     char buf[1024];
     if (nuc34_device_info(c.device, buf, sizeof(buf)) != NUC34_OK) return NUC34_E_HIP;
@@ -229,17 +260,19 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
     c.hbm_budget = static_cast<uint64_t>(0.9 * static_cast<double>(std::strtoull(f + 18, nullptr, 10)));
   }
   const uint64_t big = static_cast<uint64_t>(c.large_fraction * static_cast<double>(c.hbm_budget));
-  const uint64_t budgets[2] = {c.hbm_budget - big, big};
-  for (int li = 0; li < 2; ++li) {
-    const int rc = nuc34_ctx_create(c.device, budgets[li], &s->lanes[li].ctx);
+  s->workers.resize(1 + c.large_workers);
+  for (size_t i = 0; i < s->workers.size(); ++i) {
+    Worker& w = s->workers[i];
+    w.cls = i == 0 ? 0 : 1;
+    const uint64_t budget = i == 0 ? c.hbm_budget - big : big / c.large_workers;
+    const int rc = nuc34_ctx_create(c.device, budget, &w.ctx);
     if (rc != NUC34_OK) {
-      for (int lj = 0; lj < li; ++lj) nuc34_ctx_destroy(s->lanes[lj].ctx);
+      for (size_t k = 0; k < i; ++k) nuc34_ctx_destroy(s->workers[k].ctx);
       return rc;
     }
   }
-  s->lanes[1].batched = false;
   nuc34_server* raw = s.release();
-  for (int li = 0; li < 2; ++li) raw->lanes[li].worker = std::thread([raw, li] { raw->lane_loop(li); });
+  for (Worker& w : raw->workers) w.thread = std::thread([raw, wp = &w] { raw->work(wp); });
   *out = raw;
   return NUC34_OK;
 }
@@ -251,9 +284,9 @@ void nuc34_serve_destroy(nuc34_server* s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  for (Lane& L : s->lanes) {
-    if (L.worker.joinable()) L.worker.join();
-    nuc34_ctx_destroy(L.ctx);
+  for (Worker& w : s->workers) {
+    if (w.thread.joinable()) w.thread.join();
+    nuc34_ctx_destroy(w.ctx);
   }
   delete s;
 }
@@ -276,7 +309,7 @@ int nuc34_serve_submit(nuc34_server* s, const uint32_t* O, const uint32_t* E, ui
   *job = j->id;
   Job* raw = j.get();
   s->jobs.emplace(raw->id, std::move(j));
-  s->lanes[m >= s->cfg.large_edges ? 1 : 0].queue.push_back(raw);
+  s->queue[m >= s->cfg.large_edges ? 1 : 0].push_back(raw);
   s->st.submitted++;
   s->cv_work.notify_all();
   return NUC34_OK;

@@ -132,6 +132,17 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
   }
 }
 
+__global__ void k_tri_abc(const uint32_t* __restrict__ triA, const uint64_t* __restrict__ triBC, uint64_t count,
+                          uint32_t* __restrict__ abc) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t bc = triBC[i];
+    abc[3 * i] = triA[i];
+    abc[3 * i + 1] = static_cast<uint32_t>(bc >> 32);
+    abc[3 * i + 2] = static_cast<uint32_t>(bc);
+  }
+}
+
 unsigned tri_grid(uint64_t edges) {
   const uint64_t tiles = (edges + kWave - 1) / kWave;
   return grid_for(tiles, kWavesPerBlock, 1u << 15);
@@ -153,6 +164,12 @@ void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, 
   if (e_end <= e_begin) return;
   hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
                      e_end, rows, inpos, triA, triBC, cIdx);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_tri_abc, dim3(grid_for(count, 256, 1u << 15)), dim3(256), 0, s, triA, triBC, count, abc);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -32,7 +32,7 @@ class Nuc34Error(RuntimeError):
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("large_fraction", C.c_double),
                 ("large_edges", C.c_uint64), ("max_batch_edges", C.c_uint64), ("max_batch_graphs", C.c_uint32),
-                ("variant", C.c_int), ("batch_window_ms", C.c_double)]
+                ("variant", C.c_int), ("batch_window_ms", C.c_double), ("large_workers", C.c_uint32)]
 
 
 class JobInfo(C.Structure):
@@ -366,13 +366,15 @@ def decompose_graph(g: HostGraph, variant="auto", parts: int = 1, device: int = 
 class Server:
     """Decomposition service (nuc34_serve_*): submit undirected graphs from any
     thread, wait for and take their results. Small jobs are batched as disjoint
-    unions on one lane, large ones run alone on a second lane; both lanes share
-    the device through their own streams."""
+    unions by one worker, large ones run alone on `large_workers` workers; the
+    workers share the device through their own contexts and streams."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, large_fraction: float = 0.0, large_edges: int = 0,
-                 max_batch_edges: int = 0, max_batch_graphs: int = 0, variant="auto", batch_window_ms: float = 0.0):
+                 max_batch_edges: int = 0, max_batch_graphs: int = 0, variant="auto", batch_window_ms: float = 0.0,
+                 large_workers: int = 0):
         cfg = ServeConfig(device, hbm_budget, large_fraction, large_edges, max_batch_edges, max_batch_graphs,
-                          VARIANTS[variant] if isinstance(variant, str) else int(variant), batch_window_ms)
+                          VARIANTS[variant] if isinstance(variant, str) else int(variant), batch_window_ms,
+                          large_workers)
         self._h = C.c_void_p()
         rc = _lib.nuc34_serve_create(C.byref(cfg), C.byref(self._h))
         if rc != OK:

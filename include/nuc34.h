@@ -172,28 +172,29 @@ int nuc34_device_info(int device, char* buf, size_t len);
 /* ---- serving loop -------------------------------------------------------- */
 /*
  * No counterpart in the reference (one graph per process): a decomposition
- * service for a stream of independent graphs. Any thread submits jobs; two
- * worker lanes, each with its own context and stream on the device, take them
- * in arrival order. The small-job lane packs every queued job that fits its
+ * service for a stream of independent graphs. Any thread submits jobs; worker
+ * threads, each with its own context and stream on the device, take them in
+ * arrival order: one small-job lane and `large_workers` large-job workers. The small-job lane packs every queued job that fits its
  * caps into one disjoint-union batch (one pass of every kernel, shared peel
  * rounds) as soon as it is free; the large-job lane runs graphs at or above
  * `large_edges` one at a time, so they never hold small jobs back. Admission
- * is by HBM footprint: each batch's variant is chosen after the support count
- * (direct when its K4 lists fit the lane's budget, else on-demand), and a
- * batch that still runs out of its budget is split in halves and retried.
+ * is by HBM footprint: each lane holds its share of the budget, a batch runs
+ * in the configured variant (NUC34_AUTO: on demand, O(T) memory), and a batch
+ * that runs out of its lane's budget is split in halves and retried.
  */
 typedef struct nuc34_server nuc34_server;
 
 typedef struct nuc34_serve_config {
   int device;
-  uint64_t hbm_budget;       /* bytes for both lanes (0 = 90% of the free device memory) */
-  double large_fraction;     /* share of the budget for the large-job lane (0 = 0.75) */
+  uint64_t hbm_budget;       /* bytes for all workers (0 = 90% of the free device memory) */
+  double large_fraction;     /* share of the budget for the large-job workers (0 = 0.75) */
   uint64_t large_edges;      /* undirected edge entries (2 per edge) at which a job is large (0 = 4 Mi) */
   uint64_t max_batch_edges;  /* edge entries per small-job batch (0 = 256 Mi) */
   uint32_t max_batch_graphs; /* graphs per small-job batch (0 = 4096) */
   int variant;               /* NUC34_AUTO (default), NUC34_DIRECT or NUC34_ONDEMAND */
   double batch_window_ms;    /* small-job lane: wait this long after its oldest queued job
                                 arrived before starting a batch (0 = start at once) */
+  uint32_t large_workers;    /* workers (contexts, streams) for large jobs (0 = 2) */
 } nuc34_serve_config;
 
 typedef struct nuc34_job_info {
