@@ -596,6 +596,52 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
 
 const char* nuc34_last_error(const nuc34_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+}  // extern "C"
+
+namespace {
+
+// Installs an uploaded CSR (device buffers, already checked) as the context's
+// graph: used as given when oriented, else oriented on the device.
+int install_graph(nuc34_ctx* c, DevBuf& Ou, DevBuf& Eu, uint32_t n, uint64_t m, bool oriented) {
+  c->n = n;
+  if (oriented) {
+    c->m = m;
+    c->O = std::move(Ou);
+    c->E = std::move(Eu);
+    c->src = c->alloc.get(m * 4 + 4, "edge sources");
+    launch_fill_src(c->O.as<uint32_t>(), n, c->src.as<uint32_t>(), c->s);
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    c->st.ms_orient = 0;
+  } else {
+    if (m % 2) throw std::invalid_argument("undirected CSR must hold both directions of every edge");
+    c->st.ms_orient = timed(c, [&] {
+      DevBuf deg = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "out-degrees");
+      launch_orient_count(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, deg.as<uint32_t>(), c->s);
+      c->O = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "oriented offsets");
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(n) * 4, "scan scratch");
+      exclusive_scan<uint32_t, uint32_t>(deg.as<uint32_t>(), c->O.as<uint32_t>(), n, scratch.as<uint32_t>(), c->s);
+      const uint32_t mo = read_scalar<uint32_t>(c, c->O.as<uint32_t>() + n);
+      if (mo != m / 2) throw std::invalid_argument("undirected CSR is not symmetric");
+      c->m = mo;
+      c->E = c->alloc.get(c->m * 4 + 4, "oriented edges");
+      c->src = c->alloc.get(c->m * 4 + 4, "edge sources");
+      launch_orient_fill(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, c->O.as<uint32_t>(), c->E.as<uint32_t>(),
+                         c->src.as<uint32_t>(), c->s);
+    });
+  }
+  c->has_graph = true;
+  c->st.n = n;
+  c->st.m = c->m;
+  return NUC34_OK;
+}
+
+// Reads the device check flag; kNoVertex when the input passed.
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
+
+}  // namespace
+
+extern "C" {
+
 int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int oriented) {
   if (c) {
     c->batch_v.clear();
@@ -604,48 +650,28 @@ int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t 
   if (!c || !O || (m && !E)) return fail(c, NUC34_E_INVALID, "null argument");
   if (O[0] != 0 || O[n] != m) return fail(c, NUC34_E_INVALID, "offsets do not span the edge array");
   if (m >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "more than 2^32-1 edges");
-  {
-    const std::string bad = validate_csr(O, E, n, oriented != 0);
+  if (oriented) {  // the acyclicity check runs on the host
+    const std::string bad = validate_csr(O, E, n, true);
     if (!bad.empty()) return fail(c, NUC34_E_INVALID, bad);
   }
   return guard(c, [&]() -> int {
     reset_stages(c, 0);
     c->st = nuc34_stats{};
-    c->n = n;
     DevBuf Ou = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "input offsets");
     DevBuf Eu = c->alloc.get(m * 4 + 4, "input edges");
     NUC34_HIP(hipMemcpyAsync(Ou.as<void>(), O, (static_cast<uint64_t>(n) + 1) * 4, hipMemcpyHostToDevice, c->s));
     if (m) NUC34_HIP(hipMemcpyAsync(Eu.as<void>(), E, m * 4, hipMemcpyHostToDevice, c->s));
-    if (oriented) {
-      c->m = m;
-      c->O = std::move(Ou);
-      c->E = std::move(Eu);
-      c->src = c->alloc.get(m * 4 + 4, "edge sources");
-      launch_fill_src(c->O.as<uint32_t>(), n, c->src.as<uint32_t>(), c->s);
-      NUC34_HIP(hipStreamSynchronize(c->s));
-      c->st.ms_orient = 0;
-    } else {
-      if (m % 2) return fail(c, NUC34_E_INVALID, "undirected CSR must hold both directions of every edge");
-      c->st.ms_orient = timed(c, [&] {
-        DevBuf deg = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "out-degrees");
-        launch_orient_count(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, deg.as<uint32_t>(), c->s);
-        c->O = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "oriented offsets");
-        DevBuf scratch = c->alloc.get(scan_scratch_elems(n) * 4, "scan scratch");
-        exclusive_scan<uint32_t, uint32_t>(deg.as<uint32_t>(), c->O.as<uint32_t>(), n, scratch.as<uint32_t>(),
-                                           c->s);
-        const uint32_t mo = read_scalar<uint32_t>(c, c->O.as<uint32_t>() + n);
-        if (mo != m / 2) throw std::invalid_argument("undirected CSR is not symmetric");
-        c->m = mo;
-        c->E = c->alloc.get(c->m * 4 + 4, "oriented edges");
-        c->src = c->alloc.get(c->m * 4 + 4, "edge sources");
-        launch_orient_fill(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, c->O.as<uint32_t>(), c->E.as<uint32_t>(),
-                           c->src.as<uint32_t>(), c->s);
-      });
+    if (!oriented) {
+      DevBuf bad = c->alloc.get(4, "check flag");
+      launch_fill_u32(bad.as<uint32_t>(), 1, kNoVertex, c->s);
+      launch_check_csr(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, m, bad.as<uint32_t>(), c->s);
+      const uint32_t v = read_scalar<uint32_t>(c, bad.as<uint32_t>());
+      if (v != kNoVertex)
+        return fail(c, NUC34_E_INVALID, "vertex " + std::to_string(v) +
+                                            ": offsets out of order or range, id out of range, self-loop, or list "
+                                            "not strictly increasing");
     }
-    c->has_graph = true;
-    c->st.n = n;
-    c->st.m = c->m;
-    return NUC34_OK;
+    return install_graph(c, Ou, Eu, n, m, oriented != 0);
   });
 }
 
@@ -659,36 +685,73 @@ int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint
     c->batch_e = {0, c->m};
     return NUC34_OK;
   }
-  std::vector<uint32_t> Ou, Eu;
-  std::vector<uint64_t> vfirst(count + 1, 0), efirst_in(count + 1, 0);
+  std::vector<uint64_t> vfirst(count + 1, 0), efirst(count + 1, 0);
   for (uint32_t g = 0; g < count; ++g) {
     vfirst[g + 1] = vfirst[g] + n[g];
-    efirst_in[g + 1] = efirst_in[g] + m[g];
+    efirst[g + 1] = efirst[g] + m[g];
   }
-  if (vfirst[count] >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 vertices");
-  if (efirst_in[count] >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 edges");
-  Ou.resize(vfirst[count] + 1);
-  Eu.resize(efirst_in[count]);
-  const uint32_t* Og = O;
+  const uint64_t nv = vfirst[count], ne = efirst[count];
+  if (nv >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 vertices");
+  if (ne >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 edges");
   for (uint32_t g = 0; g < count; ++g) {
-    if (Og[0] != 0 || Og[n[g]] != m[g]) return fail(c, NUC34_E_INVALID, "graph " + std::to_string(g) + ": offsets do not span its edges");
-    for (uint32_t v = 0; v < n[g]; ++v) Ou[vfirst[g] + v] = static_cast<uint32_t>(efirst_in[g] + Og[v]);
-    const uint32_t vb = static_cast<uint32_t>(vfirst[g]);
-    for (uint64_t i = 0; i < m[g]; ++i) Eu[efirst_in[g] + i] = E[efirst_in[g] + i] + vb;
-    Og += n[g] + 1;
+    const uint32_t* Og = O + vfirst[g] + g;
+    if (Og[0] != 0 || Og[n[g]] != m[g])
+      return fail(c, NUC34_E_INVALID, "graph " + std::to_string(g) + ": offsets do not span its edges");
   }
-  Ou[vfirst[count]] = static_cast<uint32_t>(efirst_in[count]);
-  const int rc = nuc34_load_csr(c, Ou.data(), Eu.empty() ? Ou.data() : Eu.data(),
-                                static_cast<uint32_t>(vfirst[count]), efirst_in[count], oriented);
-  if (rc != NUC34_OK) return rc;
+  if (oriented) {  // rare: build the union on the host, where the acyclicity check runs
+    std::vector<uint32_t> Ou(nv + 1), Eu(ne);
+    for (uint32_t g = 0; g < count; ++g) {
+      const uint32_t* Og = O + vfirst[g] + g;
+      for (uint32_t v = 0; v < n[g]; ++v) Ou[vfirst[g] + v] = static_cast<uint32_t>(efirst[g] + Og[v]);
+      for (uint64_t i = 0; i < m[g]; ++i) Eu[efirst[g] + i] = E[efirst[g] + i] + static_cast<uint32_t>(vfirst[g]);
+    }
+    Ou[nv] = static_cast<uint32_t>(ne);
+    const int rc = nuc34_load_csr(c, Ou.data(), Eu.empty() ? Ou.data() : Eu.data(), static_cast<uint32_t>(nv), ne, 1);
+    if (rc != NUC34_OK) return rc;
+  } else {
+    const int rc = guard(c, [&]() -> int {
+      c->batch_v.clear();
+      c->batch_e.clear();
+      reset_stages(c, 0);
+      c->st = nuc34_stats{};
+      DevBuf Oin = c->alloc.get((nv + count) * 4, "batch offsets");
+      DevBuf Ein = c->alloc.get(ne * 4 + 4, "batch edges");
+      DevBuf firsts = c->alloc.get((count + 1) * 16, "batch boundaries");
+      NUC34_HIP(hipMemcpyAsync(Oin.as<void>(), O, (nv + count) * 4, hipMemcpyHostToDevice, c->s));
+      if (ne) NUC34_HIP(hipMemcpyAsync(Ein.as<void>(), E, ne * 4, hipMemcpyHostToDevice, c->s));
+      NUC34_HIP(hipMemcpyAsync(firsts.as<void>(), vfirst.data(), (count + 1) * 8, hipMemcpyHostToDevice, c->s));
+      NUC34_HIP(hipMemcpyAsync(firsts.as<uint64_t>() + count + 1, efirst.data(), (count + 1) * 8,
+                               hipMemcpyHostToDevice, c->s));
+      DevBuf Ou = c->alloc.get((nv + 1) * 4, "input offsets");
+      DevBuf Eu = c->alloc.get(ne * 4 + 4, "input edges");
+      DevBuf bad = c->alloc.get(4, "check flag");
+      launch_fill_u32(bad.as<uint32_t>(), 1, kNoVertex, c->s);
+      launch_batch_union(Oin.as<uint32_t>(), Ein.as<uint32_t>(), firsts.as<uint64_t>(),
+                         firsts.as<uint64_t>() + count + 1, count, nv, Ou.as<uint32_t>(), Eu.as<uint32_t>(),
+                         bad.as<uint32_t>(), c->s);
+      const uint32_t v = read_scalar<uint32_t>(c, bad.as<uint32_t>());
+      if (v != kNoVertex) {
+        const uint32_t g = static_cast<uint32_t>(std::upper_bound(vfirst.begin(), vfirst.end(), v) - vfirst.begin()) - 1;
+        return fail(c, NUC34_E_INVALID, "graph " + std::to_string(g) + ", vertex " + std::to_string(v - vfirst[g]) +
+                                            ": offsets out of order or range, id out of range, self-loop, or list "
+                                            "not strictly increasing");
+      }
+      Oin.release();
+      Ein.release();
+      return install_graph(c, Ou, Eu, static_cast<uint32_t>(nv), ne, false);
+    });
+    if (rc != NUC34_OK) return rc;
+  }
   return guard(c, [&]() -> int {
     // oriented edge boundaries: the union's oriented offsets at each graph's first vertex
-    std::vector<uint32_t> Oo(vfirst[count] + 1);
-    NUC34_HIP(hipMemcpyAsync(Oo.data(), c->O.as<void>(), Oo.size() * 4, hipMemcpyDeviceToHost, c->s));
+    DevBuf idx = c->alloc.get((count + 1) * 8, "batch boundaries");
+    DevBuf out = c->alloc.get((count + 1) * 8, "batch edge boundaries");
+    NUC34_HIP(hipMemcpyAsync(idx.as<void>(), vfirst.data(), (count + 1) * 8, hipMemcpyHostToDevice, c->s));
+    launch_gather_u32(c->O.as<uint32_t>(), idx.as<uint64_t>(), out.as<uint64_t>(), count + 1, c->s);
+    c->batch_e.resize(count + 1);
+    NUC34_HIP(hipMemcpyAsync(c->batch_e.data(), out.as<void>(), (count + 1) * 8, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     c->batch_v = vfirst;
-    c->batch_e.resize(count + 1);
-    for (uint32_t g = 0; g <= count; ++g) c->batch_e[g] = Oo[vfirst[g]];
     return NUC34_OK;
   });
 }

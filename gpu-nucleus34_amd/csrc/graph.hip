@@ -87,6 +87,67 @@ __global__ void k_fill(T* __restrict__ a, uint64_t n, T v) {
     a[i] = v;
 }
 
+// Undirected CSR check, thread per vertex: offsets ordered and within m, each
+// list strictly increasing with ids < n and no self-loop; *bad = lowest
+// offending vertex. A list is read only after its offsets passed, so every
+// read stays inside O[0..n] and E[0..m).
+__global__ void k_check_csr(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E, uint32_t n, uint64_t m,
+                            uint32_t* __restrict__ bad) {
+  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t o0 = O[u], o1 = O[u + 1];
+    bool ok = o0 <= o1 && o1 <= m;
+    for (uint32_t i = o0; ok && i < o1; ++i) {
+      const uint32_t x = E[i];
+      ok = x < n && x != u && (i == o0 || E[i - 1] < x);
+    }
+    if (!ok) atomic_min_u32(bad, static_cast<uint32_t>(u));
+  }
+}
+
+// Disjoint union of a batch, thread per union vertex v (graph g, local id u):
+// graph g's offsets sit at O[vfirst[g] + g ..] and its edges at E[efirst[g] ..]
+// with local ids; Ou[v] = efirst[g] + O_g[u] and every id gains vfirst[g].
+// The same checks as k_check_csr, per graph (O_g[0] = 0 and O_g[n_g] = m_g are
+// checked on the host); *bad = lowest offending union vertex.
+__global__ void k_batch_union(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
+                              const uint64_t* __restrict__ vfirst, const uint64_t* __restrict__ efirst, uint32_t count,
+                              uint64_t nv, uint32_t* __restrict__ Ou, uint32_t* __restrict__ Eu,
+                              uint32_t* __restrict__ bad) {
+  for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < nv;
+       v += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t lo = 0, hi = count;  // largest g with vfirst[g] <= v
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (vfirst[mid] <= v) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t g = lo;
+    const uint64_t u = v - vfirst[g];
+    const uint64_t ng = vfirst[g + 1] - vfirst[g], mg = efirst[g + 1] - efirst[g];
+    const uint32_t* Og = O + vfirst[g] + g;
+    const uint32_t o0 = Og[u], o1 = Og[u + 1];
+    bool ok = o0 <= o1 && o1 <= mg;
+    Ou[v] = static_cast<uint32_t>(efirst[g] + (ok ? o0 : 0));
+    const uint32_t* Eg = E + efirst[g];
+    uint32_t* Eo = Eu + efirst[g];
+    for (uint32_t i = o0; ok && i < o1; ++i) {
+      const uint32_t x = Eg[i];
+      ok = x < ng && x != u && (i == o0 || Eg[i - 1] < x);
+      Eo[i] = static_cast<uint32_t>(x + vfirst[g]);
+    }
+    if (!ok) atomic_min_u32(bad, static_cast<uint32_t>(v));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) Ou[nv] = static_cast<uint32_t>(efirst[count]);
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint64_t* __restrict__ idx,
+                             uint64_t* __restrict__ out, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    out[i] = src[idx[i]];
+}
+
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx,
                              uint64_t* __restrict__ out, uint64_t n) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -95,6 +156,25 @@ __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint64_t* _
 }
 
 }  // namespace
+
+void launch_check_csr(const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, uint32_t* bad, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_check_csr, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, O, E, n, m, bad);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_batch_union(const uint32_t* O, const uint32_t* E, const uint64_t* vfirst, const uint64_t* efirst,
+                        uint32_t count, uint64_t nv, uint32_t* Ou, uint32_t* Eu, uint32_t* bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_union, dim3(grid_for(nv ? nv : 1, 256, 1u << 16)), dim3(256), 0, s, O, E, vfirst, efirst,
+                     count, nv, Ou, Eu, bad);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_gather_u32(const uint32_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_u32, dim3(grid_for(n, 256, 1u << 14)), dim3(256), 0, s, src, idx, out, n);
+  NUC34_LAUNCH_CHECK();
+}
 
 void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s) {
   if (!n) return;

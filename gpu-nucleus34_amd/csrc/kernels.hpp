@@ -11,6 +11,14 @@ void launch_orient_count(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, uin
 void launch_orient_fill(const uint32_t* Ou, const uint32_t* Eu, uint32_t n, const uint32_t* O, uint32_t* E,
                         uint32_t* src, hipStream_t s);
 void launch_fill_src(const uint32_t* O, uint32_t n, uint32_t* src, hipStream_t s);
+// Checks of an uploaded undirected CSR (O[0] = 0 and O[n] = m checked by the
+// caller): *bad (preset to UINT32_MAX) becomes the lowest malformed vertex.
+void launch_check_csr(const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, uint32_t* bad, hipStream_t s);
+// Disjoint union of `count` undirected graphs uploaded back to back (offsets
+// at O[vfirst[g] + g ..], edges at E[efirst[g] ..], local ids) into Ou[nv+1],
+// Eu, with the same checks per graph into *bad.
+void launch_batch_union(const uint32_t* O, const uint32_t* E, const uint64_t* vfirst, const uint64_t* efirst,
+                        uint32_t count, uint64_t nv, uint32_t* Ou, uint32_t* Eu, uint32_t* bad, hipStream_t s);
 
 // (b) triangle enumeration over oriented edges [e_begin, e_end): the bits pass
 // fills the incidence rows, the emit pass (after the row ranks) writes the
@@ -24,6 +32,7 @@ void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, 
 // abc[3i..3i+2] = (a, b, c) of triangles [0, count) of the given arrays.
 void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s);
 // out[i] = src[idx[i]] for i < n (idx in bounds of src, checked by the caller).
+void launch_gather_u32(const uint32_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s);
 void launch_gather_u64(const uint64_t* src, const uint64_t* idx, uint64_t* out, uint64_t n, hipStream_t s);
 void launch_iota(uint32_t* a, uint64_t n, hipStream_t s);
 void launch_fill_i32(int32_t* a, uint64_t n, int32_t v, hipStream_t s);

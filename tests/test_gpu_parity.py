@@ -269,3 +269,31 @@ def test_batch_matches_single_graphs(variant):
         if j in order[:3]:
             ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("host", 4)
             assert np.array_equal(k_single, ref["k"]) and np.array_equal(t_single, ref["tris"]), j
+
+
+@pytest.mark.parametrize("case", ["offsets", "range", "self_loop", "unsorted", "asymmetric"])
+def test_malformed_csr_rejected(case):
+    """Malformed undirected input is rejected by the device check (which reads
+    only inside the arrays) with E_INVALID, alone and inside a batch, and the
+    context stays usable."""
+    good = (np.array([0, 2, 4, 6], np.uint32), np.array([1, 2, 0, 2, 0, 1], np.uint32))  # a triangle
+    bad = {
+        "offsets": (np.array([0, 3, 2, 6], np.uint32), np.array([1, 2, 0, 2, 0, 1], np.uint32)),
+        "range": (np.array([0, 2, 4, 6], np.uint32), np.array([1, 7, 0, 2, 0, 1], np.uint32)),
+        "self_loop": (np.array([0, 2, 4, 6], np.uint32), np.array([0, 2, 0, 2, 0, 1], np.uint32)),
+        "unsorted": (np.array([0, 2, 4, 6], np.uint32), np.array([2, 1, 0, 2, 0, 1], np.uint32)),
+        "asymmetric": (np.array([0, 1, 3, 4], np.uint32), np.array([1, 0, 2, 0], np.uint32)),
+    }[case]
+    print(f"[malformed] {case}", flush=True)
+    with nuc34.Context() as ctx:
+        with pytest.raises(nuc34.Nuc34Error) as ei:
+            ctx.load_csr(bad[0], bad[1], False)
+        assert ei.value.code == nuc34.E_INVALID
+        with pytest.raises(nuc34.Nuc34Error) as ei:
+            ctx.load_batch([good, bad, good])
+        assert ei.value.code == nuc34.E_INVALID
+        if case != "asymmetric":  # found by the check itself, which names the graph
+            assert "graph 1" in str(ei.value)
+        ctx.load_batch([good, good])
+        st = ctx.decompose("auto")
+        assert st["triangles"] == 2 and st["four_cliques"] == 0
