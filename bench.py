@@ -192,6 +192,8 @@ def run_batched(nuc34, count: int, seed: int = 3) -> dict:
     graph's slice of the batched result must equal its own decomposition."""
     gs = [nuc34.gen_batch_graph(seed, i) for i in range(count)]
     with nuc34.Context(0) as ctx:
+        ctx.load_batch(gs)  # launch shapes tuned on the union, used by both modes
+        tuning, _ = ctx.autotune("auto")
         ctx.load_batch(gs[:8])  # warm-up of both paths (allocator cache, code objects)
         ctx.decompose("auto")
         ctx.load_graph(gs[0])
@@ -214,7 +216,8 @@ def run_batched(nuc34, count: int, seed: int = 3) -> dict:
     return {"graphs": count, "seed": seed, "vertices": int(sum(g.n for g in gs)), "edges": int(sum(g.m for g in gs) // 2),
             "triangles": stb["triangles"], "four_cliques": stb["four_cliques"], "rounds": stb["rounds"],
             "batched_s": t1 - t0, "plain_s": plain_s, "graphs_per_s_batched": count / (t1 - t0),
-            "graphs_per_s_plain": count / plain_s, "speedup": plain_s / (t1 - t0), "bit_exact_vs_plain": bool(exact)}
+            "graphs_per_s_plain": count / plain_s, "speedup": plain_s / (t1 - t0), "bit_exact_vs_plain": bool(exact),
+            "tuning": tuning}
 
 
 def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: int = 4, pool: int = 256) -> dict:
@@ -295,6 +298,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain comparison")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--serve-seconds", type=float, default=0.0,
                     help="also run the serving workload (Poisson arrivals) for this many seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -319,6 +323,7 @@ def main():
         ctx.load_graph(g)
         O, E = ctx.oriented_csr()
         gs = graph_stats(O, E)
+        tuning = None if args.no_autotune else ctx.autotune(wl["variant"])[0]
         for _ in range(args.warmup):
             ctx.reset()
             ctx.decompose(wl["variant"], wl["parts"])
@@ -368,7 +373,7 @@ def main():
         "config": {"workload": wl["desc"], "variant": wl["variant"], "parts": wl["parts"],
                    "n": gs["n"], "m_oriented": gs["m"], "triangles": st["triangles"],
                    "four_cliques": st["four_cliques"], "K_max": st["k_max"], "rounds": st["rounds"],
-                   "counts_match_reference": parity,
+                   "counts_match_reference": parity, "tuning": tuning,
                    "phase_ms": {k: round(st[k], 3) for k in ("ms_triangles", "ms_support", "ms_k4_list",
                                                            "ms_incidence", "ms_peel", "ms_total")},
                    "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()}},

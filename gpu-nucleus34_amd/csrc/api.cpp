@@ -57,6 +57,7 @@ struct nuc34_ctx {
   bool has_k = false;
   DevBuf kv;
   nuc34_stats st{};
+  nuc34_tuning tune{};  // 0 = defaults
   // per-kernel event timers
   struct KTimer {
     const char* name = nullptr;
@@ -341,13 +342,22 @@ void run_k4_lists(nuc34_ctx* c) {
     if (read_scalar<uint64_t>(c, c->koff.as<uint64_t>() + T) != 4 * Q)
       throw std::logic_error("K4 list sizes disagree with the support sum");
     c->lists = c->alloc.get(4 * Q * 12 + 12, "K4 partner lists");
-    ktime(c, K_K4_LISTS, 1, [&] { launch_k4_lists(k4src(c), c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s); });
+    ktime(c, K_K4_LISTS, 1, [&] {
+      launch_k4_lists(k4src(c), c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->tune.k4_list_grid_cap, c->s);
+    });
   });
   c->has_k4 = true;
 }
 
 // Rounds enqueued per host check of the done flag (captured once as a graph).
 constexpr int kRoundsPerLaunch = 32;
+
+PeelShape peel_shape(const nuc34_ctx* c) {
+  PeelShape sh;
+  if (c->tune.peel_debit_grid) sh.debit_grid = c->tune.peel_debit_grid;
+  if (c->tune.peel_tile_cap) sh.tile_cap = c->tune.peel_tile_cap;
+  return sh;
+}
 
 void run_peel(nuc34_ctx* c, int variant) {
   const uint64_t T = c->T;
@@ -378,13 +388,16 @@ void run_peel(nuc34_ctx* c, int variant) {
     v.LA = liveA.as<uint32_t>();
     v.LC = liveC.as<uint32_t>();
   }
-  static_assert(kRoundsPerLaunch % 2 == 0, "round parity must equal slot parity");
+  // round parity must equal slot parity: groups of even size
+  const int per_launch = c->tune.peel_rounds_per_launch ? static_cast<int>(c->tune.peel_rounds_per_launch & ~1u)
+                                                        : kRoundsPerLaunch;
+  const PeelShape shape = peel_shape(c);
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {
       if (variant == NUC34_DIRECT)
-        launch_peel_round_direct(pb, r, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->s);
+        launch_peel_round_direct(pb, r, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), shape, c->s);
       else
-        launch_peel_round_ondemand(pb, r, v, c->s);
+        launch_peel_round_ondemand(pb, r, v, shape, c->s);
     }
   };
   PeelCtl hs[2] = {};
@@ -395,7 +408,7 @@ void run_peel(nuc34_ctx* c, int variant) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {
-      enqueue_rounds(kRoundsPerLaunch);
+      enqueue_rounds(per_launch);
       if (hipStreamEndCapture(c->s, &graph) != hipSuccess ||
           hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         exec = nullptr;
@@ -404,11 +417,11 @@ void run_peel(nuc34_ctx* c, int variant) {
     } else {
       (void)hipGetLastError();
     }
-    const uint64_t max_launches = 2 * T / kRoundsPerLaunch + 16;  // every round fixes >= 1 triangle
+    const uint64_t max_launches = 2 * T / per_launch + 16;  // every second round fixes >= 1 triangle
     for (uint64_t l = 0;; ++l) {
-      ktime(c, K_PEEL, kRoundsPerLaunch, [&] {
+      ktime(c, K_PEEL, per_launch, [&] {
         if (exec) NUC34_HIP(hipGraphLaunch(exec, c->s));
-        else enqueue_rounds(kRoundsPerLaunch);
+        else enqueue_rounds(per_launch);
       });
       NUC34_HIP(hipMemcpyAsync(hs, ctlb.as<void>(), sizeof(hs), hipMemcpyDeviceToHost, c->s));
       NUC34_HIP(hipStreamSynchronize(c->s));
@@ -956,6 +969,80 @@ int nuc34_reset(nuc34_ctx* c) {
 int nuc34_get_stats(const nuc34_ctx* c, nuc34_stats* stats) {
   if (!c || !stats) return NUC34_E_INVALID;
   *stats = c->st;
+  return NUC34_OK;
+}
+
+int nuc34_set_tuning(nuc34_ctx* c, const nuc34_tuning* t) {
+  if (!c || !t) return fail(c, NUC34_E_INVALID, "null argument");
+  if (t->peel_tile_cap > 64) return fail(c, NUC34_E_INVALID, "peel_tile_cap above 64");
+  if (t->peel_rounds_per_launch == 1) return fail(c, NUC34_E_INVALID, "peel_rounds_per_launch must be even");
+  if (t->peel_debit_grid > (1u << 20) || t->k4_list_grid_cap > (1u << 20))
+    return fail(c, NUC34_E_INVALID, "grid above 2^20 workgroups");
+  c->tune = *t;
+  c->tune.peel_rounds_per_launch &= ~1u;
+  return NUC34_OK;
+}
+
+int nuc34_get_tuning(const nuc34_ctx* c, nuc34_tuning* t) {
+  if (!c || !t) return NUC34_E_INVALID;
+  *t = c->tune;
+  if (!t->peel_debit_grid) t->peel_debit_grid = PeelShape{}.debit_grid;
+  if (!t->peel_tile_cap) t->peel_tile_cap = PeelShape{}.tile_cap;
+  if (!t->peel_rounds_per_launch) t->peel_rounds_per_launch = kRoundsPerLaunch;
+  if (!t->k4_list_grid_cap) t->k4_list_grid_cap = 1u << 15;
+  return NUC34_OK;
+}
+
+int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_ms) {
+  if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
+  const int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
+  if (v != NUC34_DIRECT && v != NUC34_ONDEMAND) return fail(c, NUC34_E_INVALID, "unknown variant");
+  nuc34_tuning cur;
+  nuc34_get_tuning(c, &cur);
+  auto time_of = [&](const nuc34_tuning& t, double* ms) -> int {
+    c->tune = t;
+    *ms = 1e300;
+    for (int r = 0; r < 2; ++r) {
+      int rc = nuc34_reset(c);
+      nuc34_stats st{};
+      if (rc == NUC34_OK) rc = nuc34_decompose(c, v, 1, &st);
+      if (rc != NUC34_OK) return rc;
+      *ms = std::min(*ms, st.ms_total);
+    }
+    return NUC34_OK;
+  };
+  double cur_ms = 0;
+  int rc = time_of(cur, &cur_ms);
+  if (rc != NUC34_OK) return rc;
+  struct Knob {
+    uint32_t nuc34_tuning::*field;
+    std::vector<uint32_t> values;
+    bool direct_only, ondemand_only;
+  };
+  const Knob knobs[] = {
+      {&nuc34_tuning::peel_debit_grid, {1024, 2048, 4096, 8192}, false, false},
+      {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, true},
+      {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false},
+      {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false},
+  };
+  for (const Knob& k : knobs) {
+    if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND)) continue;
+    for (uint32_t val : k.values) {
+      if (cur.*k.field == val) continue;
+      nuc34_tuning t = cur;
+      t.*k.field = val;
+      double ms = 0;
+      rc = time_of(t, &ms);
+      if (rc != NUC34_OK) return rc;
+      if (ms < cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  }
+  c->tune = cur;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
   return NUC34_OK;
 }
 

@@ -608,7 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
 // enumerated from the live bit rows, flattened over the lanes.
-__global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
+__global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s,
+                                                            uint32_t tile_cap) {
   __shared__ TileLds lds[kWPB];
   if (in->done) return;
   const int32_t level = round_level(in);
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
-  const uint32_t G = tile_width(n, static_cast<uint32_t>(nw));
+  const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
@@ -637,7 +638,6 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
     s.LC[i] = s.rows.Cm[i].bits;
 }
 
-constexpr unsigned kRoundGrid = 2048;
 constexpr unsigned kScanGrid = 512;
 
 }  // namespace
@@ -648,10 +648,11 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, hipStream_t st) {
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st) {
   if (!s.T) return;
   const uint64_t tiles = (s.T + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for(tiles, kWPB, 1u << 15)), dim3(kBlock), 0, st, s, koff, lists);
+  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for(tiles, kWPB, grid_cap ? grid_cap : 1u << 15)), dim3(kBlock), 0, st, s,
+                     koff, lists);
   NUC34_LAUNCH_CHECK();
 }
 
@@ -682,18 +683,19 @@ void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
 }
 
 void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
-                              hipStream_t s) {
+                              const PeelShape& shape, hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
-  hipLaunchKernelGGL(k_peel_debit_lists, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, out, koff, lists);
+  hipLaunchKernelGGL(k_peel_debit_lists, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, koff, lists);
   hipLaunchKernelGGL(k_peel_finish<false>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, K4Src{});
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s) {
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, const PeelShape& shape, hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
-  hipLaunchKernelGGL(k_peel_debit_bits, dim3(kRoundGrid), dim3(kBlock), 0, s, p, in, out, src);
+  hipLaunchKernelGGL(k_peel_debit_bits, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, src,
+                     static_cast<uint32_t>(shape.tile_cap ? shape.tile_cap : 1));
   hipLaunchKernelGGL(k_peel_finish<true>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, src);
   NUC34_LAUNCH_CHECK();
 }

@@ -34,7 +34,7 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
 // fourth vertex ranks above t's lowest vertex first, in order of its position
 // in that vertex's out-list, then the others in order of their fourth vertex's
 // position in the lowest vertex's in-list.
-void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, hipStream_t st);
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st);
 // Owned K4s: (t << 32 | position of the fourth vertex in N+(u)) at k4off[t] ..
 void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t* k4owned, hipStream_t st);
 // The K4 list in canonical order (by the triangle on the three lowest vertices,
@@ -90,9 +90,15 @@ struct PeelBufs {
 // round's control says done, so extra rounds are no-ops. The direct round
 // reads the K4 lists; the on-demand round enumerates each dying triangle's K4s
 // from the bit rows.
+// Launch shape of the rounds: workgroups of the debit kernel and, on demand,
+// the largest number of dying triangles one wave takes at a time.
+struct PeelShape {
+  unsigned debit_grid = 2048;
+  unsigned tile_cap = 64;
+};
 void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
-                              hipStream_t s);
-void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, hipStream_t s);
+                              const PeelShape& shape, hipStream_t s);
+void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, const PeelShape& shape, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
 // LA[i] = Aout[i] | Ain[i].bits for i < WA, LC[i] = Cm[i].bits for i < WC.
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s);

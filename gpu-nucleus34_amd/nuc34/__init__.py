@@ -52,6 +52,14 @@ class ServeStats(C.Structure):
                 ("batches", C.c_uint64), ("splits", C.c_uint64), ("busy_s", C.c_double * 2)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("peel_debit_grid", C.c_uint32), ("peel_tile_cap", C.c_uint32),
+                ("peel_rounds_per_launch", C.c_uint32), ("k4_list_grid_cap", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("ms", C.c_double), ("launches", C.c_uint64)]
 
@@ -108,6 +116,9 @@ def _load() -> C.CDLL:
         "nuc34_load_batch": (C.c_int, [vp, C.c_uint32, u32p, u64p, u32p, u32p, C.c_int]),
         "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
         "nuc34_batch_slices": (C.c_int, [vp, u64p, u64p]),
+        "nuc34_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "nuc34_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "nuc34_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
         "nuc34_serve_create": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(vp)]),
         "nuc34_serve_destroy": (None, [vp]),
@@ -344,6 +355,26 @@ class Context:
         out = np.empty((max(count, 1), 4), dtype=np.uint32)
         self._check(_lib.nuc34_copy_fourcliques(self._h, first, count, _u32(out)))
         return out[:count]
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(_lib.nuc34_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **fields):
+        t = Tuning()
+        _lib.nuc34_get_tuning(self._h, C.byref(t))
+        for k, v in fields.items():
+            setattr(t, k, v)
+        self._check(_lib.nuc34_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, variant="auto"):
+        """Times candidate launch shapes on the loaded graph and keeps the
+        fastest; returns (tuning dict, decomposition ms)."""
+        t, ms = Tuning(), C.c_double()
+        v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        self._check(_lib.nuc34_autotune(self._h, v, C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
 
     def reset(self):
         """Drop every stage after the loaded graph (buffers go to the reuse cache)."""

@@ -166,6 +166,23 @@ int nuc34_get_stats(const nuc34_ctx* ctx, nuc34_stats* stats);
  * for repeated runs (benchmarks, serving). */
 int nuc34_reset(nuc34_ctx* ctx);
 
+/* Launch shapes a context uses; 0 in any field selects its default. */
+typedef struct nuc34_tuning {
+  uint32_t peel_debit_grid;        /* workgroups of a peel round's debit kernel (2048) */
+  uint32_t peel_tile_cap;          /* on-demand: most dying triangles one wave takes at a time, 1..64 (64) */
+  uint32_t peel_rounds_per_launch; /* peel rounds per captured graph launch, even (32) */
+  uint32_t k4_list_grid_cap;       /* most workgroups of the direct K4 list build (32768) */
+} nuc34_tuning;
+
+int nuc34_set_tuning(nuc34_ctx* ctx, const nuc34_tuning* tuning);
+int nuc34_get_tuning(const nuc34_ctx* ctx, nuc34_tuning* tuning);
+/* Times candidate launch shapes on the loaded graph, one parameter at a time
+ * from the current setting (each candidate: two whole decompositions in
+ * `variant`, the faster counts), keeps the fastest in the context and returns
+ * it with its decomposition time in milliseconds. The reference launches with
+ * fixed shapes (MAX_THRD_BLK, nucleus/utils.cuh:29); no counterpart there. */
+int nuc34_autotune(nuc34_ctx* ctx, int variant, nuc34_tuning* best, double* best_ms);
+
 /* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
 int nuc34_device_info(int device, char* buf, size_t len);
 

@@ -297,3 +297,27 @@ def test_malformed_csr_rejected(case):
         ctx.load_batch([good, good])
         st = ctx.decompose("auto")
         assert st["triangles"] == 2 and st["four_cliques"] == 0
+
+
+def test_autotune_and_launch_shapes_keep_results(tmp_path):
+    """Launch shapes change speed, never results: the autotuned shape and odd
+    explicit shapes give the same k and round count in both variants."""
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 11, out=tmp_path / "g.edges")
+    g = nuc34.load_dataset(p)
+    with nuc34.Context() as ctx:
+        ctx.load_graph(g)
+        st0 = ctx.decompose("ondemand")
+        k0 = ctx.copy_k()
+        t, ms = ctx.autotune("ondemand")
+        print(f"[autotune] {t} {ms:.3f} ms", flush=True)
+        assert ms > 0 and 1 <= t["peel_tile_cap"] <= 64 and t["peel_rounds_per_launch"] % 2 == 0
+        for variant, shape in (("ondemand", {}), ("ondemand", dict(peel_debit_grid=96, peel_tile_cap=3,
+                                                                    peel_rounds_per_launch=6)),
+                               ("direct", dict(peel_debit_grid=300, k4_list_grid_cap=17))):
+            if shape:
+                ctx.set_tuning(**shape)
+            ctx.reset()
+            st = ctx.decompose(variant)
+            assert np.array_equal(ctx.copy_k(), k0) and st["rounds"] == st0["rounds"], (variant, shape)
+        with pytest.raises(nuc34.Nuc34Error):
+            ctx.set_tuning(peel_tile_cap=65)
