@@ -19,6 +19,9 @@ Prints one JSON line with the driver's contract, plus:
                 size in the same process;
   batched       512 graphs of the seeded mixed batch decomposed as one disjoint
                 union versus one decomposition per graph (graphs/s, bit-exact);
+  roofline.atomics  the peel's support decrements (device atomics) per second
+                against the measured random-atomic rate of the device;
+  device_rates  measured copy GB/s and random 4-byte atomic / load rates;
   cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
                 the reference headers) timed on this host's cores on a bounded
                 sample (R-MAT scale 14, same generator parameters).
@@ -128,7 +131,7 @@ def run_traffic(workload: str, group: str, launches: float, tmp: str, calib_mib:
     if not os.path.exists(prof):
         return None
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-batch",
-             "--no-cpu-baseline", "--no-traffic", "--workload", workload, "--copy-calibration-mib", str(calib_mib)]
+             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--workload", workload, "--copy-calibration-mib", str(calib_mib)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(tmp, counter)
@@ -287,6 +290,18 @@ def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: in
             "server": st}
 
 
+def peel_atomics(st: dict, rates: dict | None) -> dict | None:
+    """The peel's support decrements are random device-scope atomics: their rate
+    over the peel's time against the device's measured random-atomic rate."""
+    if not st.get("debits") or not st.get("ms_peel"):
+        return None
+    got = st["debits"] / (st["ms_peel"] / 1e3) / 1e9
+    peak = rates["atomic_gops"] if rates else None
+    return {"debits_per_step": st["debits"], "achieved": got, "unit": "G atomics/s", "peak_measured": peak,
+            "frac": got / peak if peak else None,
+            "floor_ms": st["debits"] / (peak * 1e9) * 1e3 if peak else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +314,8 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-rates", action="store_true",
+                    help="skip measuring the device's copy bandwidth and random atomic / load rates")
     ap.add_argument("--serve-seconds", type=float, default=0.0,
                     help="also run the serving workload (Poisson arrivals) for this many seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -334,6 +351,7 @@ def main():
             steps.append(ctx.decompose(wl["variant"], wl["parts"]))
         t1 = time.perf_counter()
         ctx.close()
+        rates = None if args.no_rates else nuc34.device_rates(0)
         batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
         serving = (run_serving(nuc34, args.serve_seconds, args.serve_rate, args.serve_large_share)
                    if args.serve_seconds > 0 else None)
@@ -381,7 +399,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_detail": traffic,
-                     "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms},
+                     "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
+                     "atomics": peel_atomics(st, rates)},
+        "device_rates": rates,
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

@@ -434,16 +434,19 @@ __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[
 // the state read can only be seen as kAlive, which gives the same owner, and
 // the extra debit of a pushed triangle cannot push it again (its support is
 // already at most level).
-__device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3], const PeelBufs& p,
-                                         int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+__device__ __forceinline__ uint32_t debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3],
+                                             const PeelBufs& p, int32_t level, uint32_t* __restrict__ nxt,
+                                             uint32_t* __restrict__ ncount) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    if (sq[k] == kGone) return;
-    if (sq[k] == kCur && q[k] < t) return;
+    if (sq[k] == kGone) return 0;
+    if (sq[k] == kCur && q[k] < t) return 0;
   }
+  uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (sq[k] != kAlive) continue;
+    ++n;
     const int32_t old = atomic_sub_i32(&p.dd[q[k]], 1);
     if (old == level + 1) {
       atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
@@ -451,13 +454,29 @@ __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const 
       nxt[atomic_add_u32(ncount, 1u)] = q[k];
     }
   }
+  return n;
 }
 
-__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
-                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+// Returns the number of debits applied.
+__device__ __forceinline__ uint32_t destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
+                                               uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
   uint32_t sq[3];
   read_states(p, q, sq);
-  debit_k4(t, q, sq, p, level, nxt, ncount);
+  return debit_k4(t, q, sq, p, level, nxt, ncount);
+}
+
+// Adds every thread's count to *total, one atomic per workgroup. Every thread
+// of the workgroup calls it.
+__device__ __forceinline__ void block_count_to(uint32_t v, uint64_t* total) {
+  __shared__ uint64_t wsum[kWPB];
+  const uint64_t s = wave_sum_u64(v);
+  if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kWPB; ++i) t += wsum[i];
+    if (t) atomic_add_u64(total, t);
+  }
 }
 
 // Dying triangles per wave for a frontier of n: spread over every wave of the
@@ -593,6 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint32_t nw = gridDim.x * kWPB;
+  uint32_t nd = 0;
   for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
     const uint32_t t = cur[i];
     const uint64_t b = koff[t], e = koff[t + 1];
@@ -600,9 +620,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
       const uint32_t* q = lists + 3 * j;
       const uint32_t qq[3] = {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
                               __builtin_nontemporal_load(q + 2)};
-      destroy_k4(t, qq, p, level, nxt, ncount);
+      nd += destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
+  block_count_to(nd, p.debits);
   reset_next_count(out);
 }
 
@@ -621,12 +642,15 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
+  uint32_t nd = 0;
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
-    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w],
-                         [&](uint32_t tt, uint32_t, const uint32_t q[3]) { destroy_k4(tt, q, p, level, nxt, ncount); });
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w], [&](uint32_t tt, uint32_t, const uint32_t q[3]) {
+      nd += destroy_k4(tt, q, p, level, nxt, ncount);
+    });
   }
+  block_count_to(nd, p.debits);
   reset_next_count(out);
 }
 

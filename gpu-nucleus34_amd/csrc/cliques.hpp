@@ -82,6 +82,7 @@ struct PeelBufs {
   uint32_t* buf1;
   PeelCtl* ctl;     // two slots
   uint64_t T;
+  uint64_t* debits; // running count of applied support decrements
 };
 
 // One peel round at position `slot` of its group: the debit of every K4 owned

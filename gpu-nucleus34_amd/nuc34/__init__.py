@@ -52,6 +52,11 @@ class ServeStats(C.Structure):
                 ("batches", C.c_uint64), ("splits", C.c_uint64), ("busy_s", C.c_double * 2)]
 
 
+class DeviceRates(C.Structure):
+    _fields_ = [("copy_gbps", C.c_double), ("atomic_gops", C.c_double), ("load_l2_gops", C.c_double),
+                ("load_hbm_gops", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("peel_debit_grid", C.c_uint32), ("peel_tile_cap", C.c_uint32),
                 ("peel_rounds_per_launch", C.c_uint32), ("k4_list_grid_cap", C.c_uint32)]
@@ -72,6 +77,7 @@ class Stats(C.Structure):
         ("ms_orient", C.c_double), ("ms_triangles", C.c_double), ("ms_incidence", C.c_double),
         ("ms_support", C.c_double), ("ms_k4_list", C.c_double), ("ms_peel", C.c_double),
         ("ms_total", C.c_double), ("n_kernels", C.c_uint32), ("kernels", KernelStat * 8),
+        ("debits", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -117,6 +123,7 @@ def _load() -> C.CDLL:
         "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
         "nuc34_batch_slices": (C.c_int, [vp, u64p, u64p]),
         "nuc34_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "nuc34_measure_device": (C.c_int, [C.c_int, C.POINTER(DeviceRates)]),
         "nuc34_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "nuc34_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
@@ -219,6 +226,15 @@ def gen_rmat(scale, edge_factor, a, b, c, seed, planted_cliques=0, planted_size=
     if rc != OK:
         raise Nuc34Error(rc, "rmat generator failed")
     return _take_graph(h)
+
+
+def device_rates(device: int = 0) -> dict:
+    """Measured copy GB/s and random atomic / load rates (10^9 per second)."""
+    r = DeviceRates()
+    rc = _lib.nuc34_measure_device(device, C.byref(r))
+    if rc != OK:
+        raise Nuc34Error(rc, "nuc34_measure_device failed")
+    return {f: getattr(r, f) for f, _ in r._fields_}
 
 
 def device_info(device: int = 0) -> dict:

@@ -79,6 +79,7 @@ typedef struct nuc34_stats {
   double ms_total;            /* triangles .. peel (orientation excluded) */
   uint32_t n_kernels;
   nuc34_kernel_stat kernels[NUC34_MAX_KERNEL_STATS];
+  uint64_t debits;            /* support decrements the peel applied (device atomics) */
 } nuc34_stats;
 
 /* ---- host-side datasets (reference loaders, exact semantics) ------------ */
@@ -184,6 +185,21 @@ int nuc34_get_tuning(const nuc34_ctx* ctx, nuc34_tuning* tuning);
 int nuc34_autotune(nuc34_ctx* ctx, int variant, nuc34_tuning* best, double* best_ms);
 
 /* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
+
+/* Measured limits of a device (about 1 GiB of device memory, well under a
+ * second): streaming device-to-device copy in (read + write) GB/s, and 10^9
+ * random 4-byte operations per second for device-scope atomic subtractions
+ * (64 MiB array) and for loads from a 4 MiB (L2-resident) and a 256 MiB array.
+ * The peel's debits are random atomics and its ownership tests random loads,
+ * so these rates bound it together with the HBM peak. No counterpart in the
+ * reference. */
+typedef struct nuc34_device_rates {
+  double copy_gbps;
+  double atomic_gops;
+  double load_l2_gops;
+  double load_hbm_gops;
+} nuc34_device_rates;
+int nuc34_measure_device(int device, nuc34_device_rates* out);
 int nuc34_device_info(int device, char* buf, size_t len);
 
 /* ---- serving loop -------------------------------------------------------- */

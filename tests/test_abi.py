@@ -46,8 +46,9 @@ def test_struct_layouts_match_header(tmp_path):
     fields = {"nuc34_serve_config": (nuc34.ServeConfig, ["hbm_budget", "large_fraction", "variant", "batch_window_ms", "large_workers"]),
               "nuc34_job_info": (nuc34.JobInfo, ["n", "t_submit", "t_done", "error"]),
               "nuc34_serve_stats": (nuc34.ServeStats, ["splits", "busy_s"]),
-              "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels"]),
-              "nuc34_tuning": (nuc34.Tuning, ["peel_tile_cap", "k4_list_grid_cap"])}
+              "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels", "debits"]),
+              "nuc34_tuning": (nuc34.Tuning, ["peel_tile_cap", "k4_list_grid_cap"]),
+              "nuc34_device_rates": (nuc34.DeviceRates, ["atomic_gops", "load_hbm_gops"])}
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "nuc34.h"', "int main(void) {"]
     for st, (_, fs) in fields.items():
         src.append(f'printf("{st} %zu\\n", sizeof({st}));')

@@ -321,3 +321,76 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
             assert np.array_equal(ctx.copy_k(), k0) and st["rounds"] == st0["rounds"], (variant, shape)
         with pytest.raises(nuc34.Nuc34Error):
             ctx.set_tuning(peel_tile_cap=65)
+
+
+def level_peel_sim(tris, O, E):
+    """Synchronous level peel of the triangle/K4 hypergraph, brute force: each
+    round removes every live triangle whose live support is at most the level,
+    and each K4 that loses its first members debits its surviving members once.
+    `tris` holds each triangle's three edge ids in the oriented CSR (O, E).
+    Returns (k, rounds, debits, essential): `debits` counts every debit of a
+    survivor, `essential` only those that bring a survivor down to the level
+    (the rest may be skipped once it is pushed)."""
+    n = len(O) - 1
+    src = np.repeat(np.arange(n), np.diff(O.astype(np.int64)))
+    adj = [set() for _ in range(n)]
+    for u, v in zip(src.tolist(), E.tolist()):
+        adj[u].add(v)
+        adj[v].add(u)
+    tid = {}
+    for i, t in enumerate(tris.tolist()):
+        tid[tuple(sorted({int(src[e]) for e in t} | {int(E[e]) for e in t}))] = i
+    assert len(tid) == len(tris) and all(len(t) == 3 for t in tid)
+    quads = []
+    for a, b, c in tid:
+        for x in adj[a] & adj[b] & adj[c]:
+            if x > c:
+                quads.append([tid[(a, b, c)], tid[(a, b, x)], tid[(a, c, x)], tid[(b, c, x)]])
+    K = np.array(quads, dtype=np.int64).reshape(-1, 4)
+    d = np.bincount(K.ravel(), minlength=len(tris)).astype(np.int64)
+    alive = np.ones(len(tris), bool)
+    dead = np.zeros(len(K), bool)
+    k = np.zeros(len(tris), np.int64)
+    level, rounds, debits, essential = 0, 0, 0, 0
+    while alive.any():
+        level = max(level, int(d[alive].min()))
+        S = alive & (d <= level)
+        while S.any():
+            rounds += 1
+            hit = ~dead & S[K].any(axis=1)
+            dead |= hit
+            members = K[hit].ravel()
+            take = members[alive[members] & ~S[members]]
+            r = np.bincount(take, minlength=len(tris))
+            essential += int(np.minimum(r, np.maximum(d - level, 0)).sum())
+            d -= r
+            debits += len(take)
+            k[S] = level
+            alive &= ~S
+            S = alive & (d <= level)
+    return k, rounds, debits, essential
+
+
+@pytest.mark.parametrize("name", ["rmat8.edges", "rmat7k8.edges", "ba60.edges", "two_cliques.edges"])
+def test_peel_debit_count_and_rounds(name):
+    """The peel's round count equals a brute-force synchronous level peel's,
+    whose k and rounds agree with the oracle's; its debit count lies between
+    the essential debits and all debits of that peel (a survivor pushed during
+    a round may or may not see its later debits of the round, by timing)."""
+    print(f"[debits] {name}", flush=True)
+    g = nuc34.load_dataset(os.path.join(GOLDEN, name))
+    og = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True)
+    ref = og.run("ondemand", 1)
+    k, rounds, debits, essential = level_peel_sim(ref["tris"], *og.csr())
+    assert np.array_equal(k, ref["k"]) and rounds == ref["rounds"]
+    for variant in ("direct", "ondemand"):
+        _, _, kg, st = gpu_run(g, variant)
+        assert np.array_equal(kg, ref["k"]) and st["rounds"] == rounds, variant
+        assert essential <= st["debits"] <= debits, (variant, st["debits"], essential, debits)
+
+
+def test_device_rates():
+    r = nuc34.device_rates(0)
+    print(f"[rates] {r}", flush=True)
+    assert all(np.isfinite(v) and v > 0 for v in r.values())
+    assert r["copy_gbps"] > 100 and r["atomic_gops"] > 0.5
