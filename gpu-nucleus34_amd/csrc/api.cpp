@@ -378,9 +378,8 @@ void run_peel(nuc34_ctx* c, int variant) {
   DevBuf b0 = c->alloc.get(T * 4, "frontier");
   DevBuf b1 = c->alloc.get(T * 4, "frontier");
   DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
-  DevBuf debits = c->alloc.get(8, "debit count");
-  PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(), b0.as<uint32_t>(),
-              b1.as<uint32_t>(), ctlb.as<PeelCtl>(), T,                   debits.as<uint64_t>()};
+  PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
+              b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
   K4Src v = k4src(c);
   DevBuf liveA, liveC;
   if (variant != NUC34_DIRECT) {
@@ -405,7 +404,6 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl h{};
   c->st.ms_peel = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
-    NUC34_HIP(hipMemsetAsync(debits.as<void>(), 0, 8, c->s));  // before the captured rounds
     if (variant != NUC34_DIRECT) launch_live_rows(v, c->WA, c->WC, c->s);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -440,7 +438,13 @@ void run_peel(nuc34_ctx* c, int variant) {
   c->st.rounds = h.round;
   c->st.k_max = h.level;
   c->st.unpeeled = T - h.peeled;
-  c->st.debits = read_scalar<uint64_t>(c, debits.as<void>());
+  {
+    // every debit lowers one residual support by one
+    DevBuf rsum = c->alloc.get(8, "residual support sum");
+    NUC34_HIP(hipMemsetAsync(rsum.as<void>(), 0, 8, c->s));
+    launch_residual_sum(pb, rsum.as<int64_t>(), c->s);
+    c->st.debits = static_cast<uint64_t>(4 * static_cast<int64_t>(c->st.four_cliques) - read_scalar<int64_t>(c, rsum.as<void>()));
+  }
 }
 
 // Host-side checks that keep every device index in bounds: monotone offsets,

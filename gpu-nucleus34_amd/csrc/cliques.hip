@@ -434,49 +434,35 @@ __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[
 // the state read can only be seen as kAlive, which gives the same owner, and
 // the extra debit of a pushed triangle cannot push it again (its support is
 // already at most level).
-__device__ __forceinline__ uint32_t debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3],
-                                             const PeelBufs& p, int32_t level, uint32_t* __restrict__ nxt,
-                                             uint32_t* __restrict__ ncount) {
+__device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3], const PeelBufs& p,
+                                         int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    if (sq[k] == kGone) return 0;
-    if (sq[k] == kCur && q[k] < t) return 0;
+    if (sq[k] == kGone) return;
+    if (sq[k] == kCur && q[k] < t) return;
   }
-  uint32_t n = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    if (sq[k] != kAlive) continue;
-    ++n;
-    const int32_t old = atomic_sub_i32(&p.dd[q[k]], 1);
-    if (old == level + 1) {
-      atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
-      p.kv[q[k]] = level;
-      nxt[atomic_add_u32(ncount, 1u)] = q[k];
+    bool push = false;
+    if (sq[k] == kAlive) {
+      push = atomic_sub_i32(&p.dd[q[k]], 1) == level + 1;
+      if (push) {
+        atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
+        p.kv[q[k]] = level;
+      }
     }
+    // one append per wave: the frontier counter is a single hot address
+    bool got;
+    const uint32_t slot = wave_append(ncount, push, &got);
+    if (got) nxt[slot] = q[k];
   }
-  return n;
 }
 
-// Returns the number of debits applied.
-__device__ __forceinline__ uint32_t destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
-                                               uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
+                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
   uint32_t sq[3];
   read_states(p, q, sq);
-  return debit_k4(t, q, sq, p, level, nxt, ncount);
-}
-
-// Adds every thread's count to *total, one atomic per workgroup. Every thread
-// of the workgroup calls it.
-__device__ __forceinline__ void block_count_to(uint32_t v, uint64_t* total) {
-  __shared__ uint64_t wsum[kWPB];
-  const uint64_t s = wave_sum_u64(v);
-  if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kWPB; ++i) t += wsum[i];
-    if (t) atomic_add_u64(total, t);
-  }
+  debit_k4(t, q, sq, p, level, nxt, ncount);
 }
 
 // Dying triangles per wave for a frontier of n: spread over every wave of the
@@ -612,7 +598,6 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint32_t nw = gridDim.x * kWPB;
-  uint32_t nd = 0;
   for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
     const uint32_t t = cur[i];
     const uint64_t b = koff[t], e = koff[t + 1];
@@ -620,10 +605,9 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
       const uint32_t* q = lists + 3 * j;
       const uint32_t qq[3] = {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
                               __builtin_nontemporal_load(q + 2)};
-      nd += destroy_k4(t, qq, p, level, nxt, ncount);
+      destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
-  block_count_to(nd, p.debits);
   reset_next_count(out);
 }
 
@@ -642,16 +626,31 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
-  uint32_t nd = 0;
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
-    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w], [&](uint32_t tt, uint32_t, const uint32_t q[3]) {
-      nd += destroy_k4(tt, q, p, level, nxt, ncount);
-    });
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w],
+                         [&](uint32_t tt, uint32_t, const uint32_t q[3]) { destroy_k4(tt, q, p, level, nxt, ncount); });
   }
-  block_count_to(nd, p.debits);
   reset_next_count(out);
+}
+
+// Sum of the residual supports dd[0..T) into *total (which starts at zero).
+// Every debit lowers one dd by one, so the peel's debits are 4Q minus this sum.
+__global__ __launch_bounds__(kBlock) void k_residual_sum(const int32_t* __restrict__ dd, uint64_t T, int64_t* total) {
+  __shared__ int64_t wsum[kWPB];
+  int64_t acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    acc += dd[i];
+  acc = static_cast<int64_t>(wave_sum_u64(static_cast<uint64_t>(acc)));
+  if (lane_id() == 0) wsum[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < kWPB; ++i) t += wsum[i];
+    if (t) atomic_add_u64(reinterpret_cast<uint64_t*>(total), static_cast<uint64_t>(t));
+  }
 }
 
 __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
@@ -665,6 +664,12 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
 constexpr unsigned kScanGrid = 512;
 
 }  // namespace
+
+void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s) {
+  if (!p.T) return;
+  hipLaunchKernelGGL(k_residual_sum, dim3(grid_for(p.T, kBlock, 2048)), dim3(kBlock), 0, s, p.dd, p.T, total);
+  NUC34_LAUNCH_CHECK();
+}
 
 void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st) {
   if (!s.T) return;
