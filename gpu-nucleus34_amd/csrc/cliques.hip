@@ -45,14 +45,14 @@ constexpr int kWPB = kBlock / kWave;
 // A triangle's rows. Every Aout/Ain row of a vertex's out-edges has the same
 // length, so the row of (u, y) is rowA[first out-edge of u] + pos(y) * W.
 struct TriRows {
-  uint64_t ru0;     // Aout / Ain row of u's first out-edge (rows of a, b, (u,x) follow)
-  uint64_t rv0;     // Aout / Ain row of v's first out-edge
-  uint64_t ca, cb;  // Cm rows of a and b (over N-(u))
-  uint64_t cc;      // Cm row of c (over N-(v))
-  uint32_t c, ou, ov, jv, jw;
-  uint32_t W;       // words of the rows of u's out-edges
-  uint32_t Wv;      // words of the rows of v's out-edges
-  uint32_t WC;      // words of the a / b c-role rows
+  uint64_t ru0;  // Aout / Ain row of u's first out-edge (rows of a, b, (u,x) follow)
+  uint64_t rv0;  // Aout / Ain row of v's first out-edge
+  uint64_t ca;   // Cm row of a (over N-(u)); b's is (jv - jw) * WC words further
+  uint64_t cc;   // Cm row of c (over N-(v))
+  uint32_t c, ov, jv, jw;
+  uint32_t W;    // words of the rows of u's out-edges
+  uint32_t Wv;   // words of the rows of v's out-edges
+  uint32_t WC;   // words of the a / b c-role rows
 };
 
 __device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
@@ -61,10 +61,10 @@ __device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
   const uint64_t bc = s.triBC[t];
   const uint32_t b = static_cast<uint32_t>(bc >> 32);
   r.c = static_cast<uint32_t>(bc);
-  r.ou = s.O[s.src[b]];
+  const uint32_t ou = s.O[s.src[b]];
   r.ov = s.O[s.E[b]];
-  r.jv = b - r.ou;
-  r.jw = a - r.ou;
+  r.jv = b - ou;
+  r.jw = a - ou;
   const uint64_t ra = s.rows.rowA[a];
   r.W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
   r.ru0 = ra - static_cast<uint64_t>(r.jw) * r.W;
@@ -73,9 +73,32 @@ __device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
   r.rv0 = rc - static_cast<uint64_t>(r.c - r.ov) * r.Wv;
   r.ca = s.rows.rowC[a];
   r.WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - r.ca);
-  r.cb = r.ca - static_cast<uint64_t>(r.jw) * r.WC + static_cast<uint64_t>(r.jv) * r.WC;
   r.cc = s.rows.rowC[r.c];
   return r;
+}
+
+__device__ __forceinline__ uint32_t shfl_u32(uint32_t v, uint32_t src) { return __shfl(v, static_cast<int>(src)); }
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, uint32_t src) {
+  return static_cast<uint64_t>(shfl_u32(static_cast<uint32_t>(v), src)) |
+         static_cast<uint64_t>(shfl_u32(static_cast<uint32_t>(v >> 32), src)) << 32;
+}
+
+// Lane src's rows. Every lane of the wave must be active (a lane reads
+// another lane's registers only while that lane is active).
+__device__ __forceinline__ TriRows shfl_rows(const TriRows& r, uint32_t src) {
+  TriRows o;
+  o.ru0 = shfl_u64(r.ru0, src);
+  o.rv0 = shfl_u64(r.rv0, src);
+  o.ca = shfl_u64(r.ca, src);
+  o.cc = shfl_u64(r.cc, src);
+  o.c = shfl_u32(r.c, src);
+  o.ov = shfl_u32(r.ov, src);
+  o.jv = shfl_u32(r.jv, src);
+  o.jw = shfl_u32(r.jw, src);
+  o.W = shfl_u32(r.W, src);
+  o.Wv = shfl_u32(r.Wv, src);
+  o.WC = shfl_u32(r.WC, src);
+  return o;
 }
 
 // Word wi of t's K4 row (W up words, then WC down words) and the row words the
@@ -99,7 +122,9 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     k.Ib = s.rows.Ain[rb];
     k.bits = LIVE ? s.LA[ra] & s.LA[rb] : (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
   } else {
-    const uint64_t ca = r.ca + (wi - r.W), cb = r.cb + (wi - r.W);
+    // b's row: (jv - jw) * WC words from a's (either sign; 64-bit wraparound)
+    const uint64_t ca = r.ca + (wi - r.W);
+    const uint64_t cb = ca + static_cast<uint64_t>(r.jv) * r.WC - static_cast<uint64_t>(r.jw) * r.WC;
     k.Aa = k.Ab = 0;
     k.Ia = s.rows.Cm[ca];
     k.Ib = s.rows.Cm[cb];
@@ -161,35 +186,45 @@ __device__ __forceinline__ uint32_t find_slot(const uint32_t* pre, uint32_t key)
   return lo;
 }
 
-// Per-wave LDS of the tile walk.
+__device__ __forceinline__ K4Word shfl_word(const K4Word& k, uint32_t src) {
+  K4Word o;
+  o.bits = shfl_u32(k.bits, src);
+  o.Aa = shfl_u32(k.Aa, src);
+  o.Ab = shfl_u32(k.Ab, src);
+  o.Ia.bits = shfl_u32(k.Ia.bits, src);
+  o.Ia.rank = shfl_u32(k.Ia.rank, src);
+  o.Ib.bits = shfl_u32(k.Ib.bits, src);
+  o.Ib.rank = shfl_u32(k.Ib.rank, src);
+  return o;
+}
+
+// Per-wave LDS of the tile walk: the prefix arrays every lane searches. The
+// triangles' rows and the chunk's words stay in the registers of the lane
+// that loaded them and travel by cross-lane shuffles, which keeps the LDS
+// per wave small enough for eight waves per SIMD.
 struct TileLds {
-  TriRows rows[kWave];
-  uint32_t tri[kWave];
   uint32_t pre[kWave + 1];  // first word item of each triangle
   uint32_t run[kWave];      // K4s of each triangle listed so far
-  K4Word word[kWave];       // the current chunk of words ...
-  uint32_t wtri[kWave];     // ... their triangle slot,
-  uint32_t wi[kWave];       // ... word index,
-  uint32_t epre[kWave];     // ... first K4 in the chunk,
-  uint32_t lpos[kWave];     // ... and list position of their first K4
+  uint32_t epre[kWave];     // first K4 of each word of the current chunk
 };
 
-// One tile: lane l brings triangle t (when valid). LIVE: K4 bits from the live
-// rows (the on-demand peel). The tile's row words are
+// One tile: lane l brings triangle t (when valid) and a value aux of its own
+// that reaches emit with each of t's K4s. LIVE: K4 bits from the live rows
+// (the on-demand peel). The tile's row words are
 // flattened over the lanes 64 at a time, then the set bits of those 64 words
-// (the K4s) are flattened again, one K4 per lane and step, and emit(t, pos, q)
-// runs once per K4 of each triangle. With POS, pos is the K4's index in the
+// (the K4s) are flattened again, one K4 per lane and step, and
+// emit(t, aux, pos, q) runs once per K4 of each triangle. With POS, pos is the K4's index in the
 // triangle's list (up words first, word and bit order); without, pos is 0.
 template <bool POS, bool LIVE, typename Emit>
-__device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, TileLds& L, Emit&& emit) {
+__device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, uint64_t aux, TileLds& L,
+                                        Emit&& emit) {
   const unsigned lane = lane_id();
   uint32_t words = 0;
+  TriRows own{};
   if (valid) {
-    const TriRows r = tri_rows(s, t);
-    L.rows[lane] = r;
-    words = r.W + r.WC;
+    own = tri_rows(s, t);
+    words = own.W + own.WC;
   }
-  L.tri[lane] = t;
   L.run[lane] = 0;
   const uint32_t inc = wave_inclusive_sum(words);
   const uint32_t total = __shfl(inc, kWave - 1);
@@ -200,43 +235,49 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
     // 1. one word per lane
     const uint32_t item = base + lane;
     const bool act = item < total;
-    uint32_t i = 0, wi = 0;
+    const uint32_t i = find_slot(L.pre, act ? item : total - 1);
+    const TriRows ri = shfl_rows(own, i);  // every lane takes part
+    uint32_t wi = 0;
     K4Word k{};
     if (act) {
-      i = find_slot(L.pre, item);
       wi = item - L.pre[i];
-      k = k4_word<LIVE>(s, L.rows[i], wi);
+      k = k4_word<LIVE>(s, ri, wi);
     }
     const uint32_t pc = static_cast<uint32_t>(__popc(k.bits));
     const uint32_t incl = wave_inclusive_sum(pc);
     const uint32_t nk = __shfl(incl, kWave - 1);
-    L.word[lane] = k;
-    L.wtri[lane] = i;
-    L.wi[lane] = wi;
-    L.epre[lane] = incl - pc;
+    const uint32_t epre = incl - pc;
+    L.epre[lane] = epre;
+    uint32_t lpos = 0;
     if constexpr (POS) {
       // list position of the word's first K4: a segmented scan per triangle
       const uint32_t first = act ? (L.pre[i] > base ? L.pre[i] - base : 0u) : lane;
       const uint32_t before = __shfl(incl, static_cast<int>(first > 0 ? first - 1 : 0));
       const uint32_t seg0 = first > 0 ? before : 0u;
-      L.lpos[lane] = act ? L.run[i] + incl - pc - seg0 : 0u;
+      lpos = act ? L.run[i] + epre - seg0 : 0u;
       wave_sync();
       const bool last = act && (lane == kWave - 1 || item + 1 == total || item + 1 == L.pre[i + 1]);
       if (last) L.run[i] += incl - seg0;
     }
     wave_sync();
-    // 2. one K4 per lane
+    // 2. one K4 per lane; the shuffles run on every lane
     for (uint32_t eb = 0; eb < nk; eb += kWave) {
       const uint32_t e = eb + lane;
+      const uint32_t sl = find_slot(L.epre, e < nk ? e : nk - 1);
+      const K4Word kw = shfl_word(k, sl);
+      const uint32_t ti = shfl_u32(i, sl);
+      const uint32_t wsl = shfl_u32(wi, sl);
+      const uint32_t esl = shfl_u32(epre, sl);
+      const uint32_t psl = POS ? shfl_u32(lpos, sl) : 0u;
+      const TriRows rt = shfl_rows(own, ti);
+      const uint32_t tt = shfl_u32(t, ti);
+      const uint64_t ta = shfl_u64(aux, ti);
       if (e < nk) {
-        const uint32_t sl = find_slot(L.epre, e);
-        const K4Word kw = L.word[sl];
-        const uint32_t r = e - L.epre[sl];
+        const uint32_t r = e - esl;
         const uint32_t j = select_bit(kw.bits, r);
-        const uint32_t ti = L.wtri[sl];
         uint32_t q[3];
-        k4_partners(s, L.rows[ti], L.wi[sl], kw, j, q);
-        emit(L.tri[ti], POS ? L.lpos[sl] + r : 0u, q);
+        k4_partners(s, rt, wsl, kw, j, q);
+        emit(tt, ta, POS ? psl + r : 0u, q);
       }
     }
     wave_sync();
@@ -279,13 +320,16 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < s.T; tile += nw) {
     const uint64_t t = tile * kWave + lane_id();
-    k4_tile<true, false>(s, static_cast<uint32_t>(t < s.T ? t : 0), t < s.T, lds[w],
-                  [&](uint32_t tt, uint32_t pos, const uint32_t q[3]) {
-                    uint32_t* o = lists + 3 * (koff[tt] + pos);
-                    o[0] = q[0];
-                    o[1] = q[1];
-                    o[2] = q[2];
-                  });
+    const bool valid = t < s.T;
+    // the lists stream out past the cache (they are read back a round at a
+    // time much later); the row words of the lookups keep it
+    k4_tile<true, false>(s, static_cast<uint32_t>(valid ? t : 0), valid, valid ? koff[t] : 0, lds[w],
+                         [&](uint32_t, uint64_t first, uint32_t pos, const uint32_t q[3]) {
+                           uint32_t* o = lists + 3 * (first + pos);
+                           __builtin_nontemporal_store(q[0], o);
+                           __builtin_nontemporal_store(q[1], o + 1);
+                           __builtin_nontemporal_store(q[2], o + 2);
+                         });
   }
 }
 
@@ -629,8 +673,9 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
-    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, lds[w],
-                         [&](uint32_t tt, uint32_t, const uint32_t q[3]) { destroy_k4(tt, q, p, level, nxt, ncount); });
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, lds[w], [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3]) {
+      destroy_k4(tt, q, p, level, nxt, ncount);
+    });
   }
   reset_next_count(out);
 }
