@@ -228,6 +228,8 @@ std::vector<std::pair<uint64_t, uint64_t>> edge_parts(nuc34_ctx* c, uint32_t par
   return r;
 }
 
+uint32_t tri_tile(const nuc34_ctx* c) { return c->tune.tri_tile_edges ? c->tune.tri_tile_edges : 64u; }
+
 BitRows bitrows(nuc34_ctx* c) {
   return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<RankWord>(),
                  c->Cm.as<RankWord>()};
@@ -275,7 +277,7 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     ktime(c, K_TRI_BITS, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_bits(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
-                        c->inpos.as<uint32_t>(), c->s);
+                        c->inpos.as<uint32_t>(), tri_tile(c), c->s);
     });
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
@@ -303,7 +305,7 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
       for (const auto& r : ranges)
         launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
                         c->inpos.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                        c->cIdx.as<uint32_t>(), c->s);
+                        c->cIdx.as<uint32_t>(), tri_tile(c), c->s);
     });
   });
   c->has_tris = true;
@@ -356,6 +358,7 @@ PeelShape peel_shape(const nuc34_ctx* c) {
   PeelShape sh;
   if (c->tune.peel_debit_grid) sh.debit_grid = c->tune.peel_debit_grid;
   if (c->tune.peel_tile_cap) sh.tile_cap = c->tune.peel_tile_cap;
+  if (c->tune.peel_finish_grid) sh.finish_grid = c->tune.peel_finish_grid;
   return sh;
 }
 
@@ -983,8 +986,10 @@ int nuc34_set_tuning(nuc34_ctx* c, const nuc34_tuning* t) {
   if (!c || !t) return fail(c, NUC34_E_INVALID, "null argument");
   if (t->peel_tile_cap > 64) return fail(c, NUC34_E_INVALID, "peel_tile_cap above 64");
   if (t->peel_rounds_per_launch == 1) return fail(c, NUC34_E_INVALID, "peel_rounds_per_launch must be even");
-  if (t->peel_debit_grid > (1u << 20) || t->k4_list_grid_cap > (1u << 20))
+  if (t->peel_debit_grid > (1u << 20) || t->k4_list_grid_cap > (1u << 20) || t->peel_finish_grid > (1u << 20))
     return fail(c, NUC34_E_INVALID, "grid above 2^20 workgroups");
+  if (t->tri_tile_edges && t->tri_tile_edges != 16 && t->tri_tile_edges != 32 && t->tri_tile_edges != 64)
+    return fail(c, NUC34_E_INVALID, "tri_tile_edges must be 16, 32 or 64");
   c->tune = *t;
   c->tune.peel_rounds_per_launch &= ~1u;
   return NUC34_OK;
@@ -997,6 +1002,8 @@ int nuc34_get_tuning(const nuc34_ctx* c, nuc34_tuning* t) {
   if (!t->peel_tile_cap) t->peel_tile_cap = PeelShape{}.tile_cap;
   if (!t->peel_rounds_per_launch) t->peel_rounds_per_launch = kRoundsPerLaunch;
   if (!t->k4_list_grid_cap) t->k4_list_grid_cap = 1u << 15;
+  if (!t->tri_tile_edges) t->tri_tile_edges = 64;
+  if (!t->peel_finish_grid) t->peel_finish_grid = PeelShape{}.finish_grid;
   return NUC34_OK;
 }
 
@@ -1031,6 +1038,8 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
       {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, true},
       {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false},
       {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false},
+      {&nuc34_tuning::tri_tile_edges, {16, 32, 64}, false, false},
+      {&nuc34_tuning::peel_finish_grid, {256, 512, 1024}, false, false},
   };
   for (const Knob& k : knobs) {
     if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND)) continue;

@@ -616,24 +616,38 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
     out->error = error;
     out->peeled = peeled;
     out->done = done;
-    in->minval = kNoMin;  // the next round's finish reduces into this slot
-    if (done) in->done = 1;
+    // `in` is read by every workgroup of this kernel, and a late one may
+    // start after this one has ended: it is left alone here (the next debit
+    // kernel prepares it for reuse)
   }
 }
 
-// Zeroes the count the finish of this round collects into. Done after the
-// kernel's last read of its own control: a store through `out` earlier (it
-// may alias `in` as far as the compiler knows) turns every later read of the
-// round's control into a vector load.
-__device__ __forceinline__ void reset_next_count(PeelCtl* out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) out->nspec = 0;
+// Prepares the slot this round's finish writes: its collection count and its
+// min reduction start empty. Only the debit kernel touches `out` (its finish
+// reads `out`'s fields only after it), and only after the kernel's last read
+// of its own control: a store through `out` earlier (it may alias `in` as far
+// as the compiler knows) turns every later read of the round's control into a
+// vector load.
+__device__ __forceinline__ void prepare_out_slot(PeelCtl* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out->nspec = 0;
+    out->minval = kNoMin;
+  }
+}
+
+// A finished peel: rounds still queued in the captured launch do nothing, and
+// both slots end up marked done, whichever one the next round reads.
+__device__ __forceinline__ bool peel_done(const PeelCtl* in, PeelCtl* out) {
+  if (!in->done) return false;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out->done = 1;
+  return true;
 }
 
 // Direct debit: one wave per dying triangle, one K4 list entry per lane.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
                                                              const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
-  if (in->done) return;
+  if (peel_done(in, out)) return;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
@@ -652,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
       destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
-  reset_next_count(out);
+  prepare_out_slot(out);
 }
 
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
@@ -660,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s,
                                                             uint32_t tile_cap) {
   __shared__ TileLds lds[kWPB];
-  if (in->done) return;
+  if (peel_done(in, out)) return;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
@@ -677,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
       destroy_k4(tt, q, p, level, nxt, ncount);
     });
   }
-  reset_next_count(out);
+  prepare_out_slot(out);
 }
 
 // Sum of the residual supports dd[0..T) into *total (which starts at zero).
@@ -705,8 +719,6 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < WC; i += stride)
     s.LC[i] = s.rows.Cm[i].bits;
 }
-
-constexpr unsigned kScanGrid = 512;
 
 }  // namespace
 
@@ -761,7 +773,7 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
   hipLaunchKernelGGL(k_peel_debit_lists, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, koff, lists);
-  hipLaunchKernelGGL(k_peel_finish<false>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, K4Src{});
+  hipLaunchKernelGGL(k_peel_finish<false>, dim3(shape.finish_grid), dim3(kBlock), 0, s, p, in, out, K4Src{});
   NUC34_LAUNCH_CHECK();
 }
 
@@ -770,7 +782,7 @@ void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, c
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
   hipLaunchKernelGGL(k_peel_debit_bits, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, src,
                      static_cast<uint32_t>(shape.tile_cap ? shape.tile_cap : 1));
-  hipLaunchKernelGGL(k_peel_finish<true>, dim3(kScanGrid), dim3(kBlock), 0, s, p, in, out, src);
+  hipLaunchKernelGGL(k_peel_finish<true>, dim3(shape.finish_grid), dim3(kBlock), 0, s, p, in, out, src);
   NUC34_LAUNCH_CHECK();
 }
 

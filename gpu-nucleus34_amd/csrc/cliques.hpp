@@ -95,15 +95,16 @@ struct PeelBufs {
 struct PeelShape {
   unsigned debit_grid = 2048;
   unsigned tile_cap = 64;
+  unsigned finish_grid = 512;  // any size is correct: the finish loops are grid-strided
 };
 void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff, const uint32_t* lists,
                               const PeelShape& shape, hipStream_t s);
 void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, const PeelShape& shape, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
-// LA[i] = Aout[i] | Ain[i].bits for i < WA, LC[i] = Cm[i].bits for i < WC.
 // Sum of the residual supports after the peel into *total (zeroed by the
 // caller): the peel applied 4Q minus this many support decrements.
 void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s);
+// LA[i] = Aout[i] | Ain[i].bits for i < WA, LC[i] = Cm[i].bits for i < WC.
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s);
 
 }  // namespace nuc34

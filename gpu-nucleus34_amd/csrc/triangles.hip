@@ -4,9 +4,10 @@
 // cliques_tri_par (nucleus/cliques.hpp:98-226): for each oriented edge b=(u,v),
 // every w in N+(u) ∩ N+(v) gives the triangle {a = idx(u,w), b, c = idx(v,w)}.
 //
-// Here one wave takes a tile of 64 consecutive oriented edges and flattens the
-// work of the tile (sum of d+(v) over its edges) across its 64 lanes, so lane
-// utilisation does not depend on list lengths. The tile's owner vertices are
+// Here one wave takes a tile of G consecutive oriented edges (G = 16, 32 or 64,
+// a launch-shape choice) and flattens the work of the tile (sum of d+(v) over
+// its edges) across its 64 lanes, so lane utilisation does not depend on list
+// lengths; smaller tiles stage shorter owner ranges. The tile's owner vertices are
 // consecutive, so their out-lists are one contiguous range of E, staged in LDS
 // when it fits; each candidate w = E[c] is then found in N+(u) by binary search
 // in LDS (global memory otherwise).
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
                                                      uint64_t e_end, BitRows rows,
                                                      const uint32_t* __restrict__ inpos,
                                                      uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                                     uint32_t* __restrict__ cIdx) {
+                                                     uint32_t* __restrict__ cIdx, uint32_t G) {
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
   __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
   __shared__ uint32_t s_len[kWavesPerBlock][kWave];
@@ -54,11 +55,11 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + w; e_begin + tile * kWave < e_end;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + w; e_begin + tile * G < e_end;
        tile += nwaves) {
-    const uint64_t e0 = e_begin + tile * kWave;
+    const uint64_t e0 = e_begin + tile * G;
     const uint64_t e = e0 + lane;
-    const bool valid = e < e_end;
+    const bool valid = lane < G && e < e_end;
     uint32_t work = 0, ob = 0, oe = 0, vb = 0;
     if (valid) {
       const uint32_t u = src[e];
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
     }
     const uint32_t inc = wave_inclusive_sum(work);
     const uint32_t total = __shfl(inc, kWave - 1);
-    const uint64_t last_lane = (e_end - e0 < static_cast<uint64_t>(kWave) ? e_end - e0 : kWave) - 1;
+    const uint64_t last_lane = (e_end - e0 < static_cast<uint64_t>(G) ? e_end - e0 : G) - 1;
     const uint32_t lo = __shfl(ob, 0);
     const uint32_t hi = __shfl(oe, static_cast<int>(last_lane));
     const bool staged = (hi - lo) <= kListCap;
@@ -143,27 +144,34 @@ __global__ void k_tri_abc(const uint32_t* __restrict__ triA, const uint64_t* __r
   }
 }
 
-unsigned tri_grid(uint64_t edges) {
-  const uint64_t tiles = (edges + kWave - 1) / kWave;
+unsigned tri_grid(uint64_t edges, uint32_t G) {
+  const uint64_t tiles = (edges + G - 1) / G;
   return grid_for(tiles, kWavesPerBlock, 1u << 15);
+}
+
+uint32_t tile_edges(uint32_t G) {
+  if (G != 16 && G != 32 && G != 64) throw std::logic_error("triangle tile width must be 16, 32 or 64");
+  return G;
 }
 
 }  // namespace
 
 void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint32_t* inpos, hipStream_t s) {
+                     const BitRows& rows, const uint32_t* inpos, uint32_t tile, hipStream_t s) {
   if (e_end <= e_begin) return;
-  hipLaunchKernelGGL(k_tri_enum<false>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, inpos, nullptr, nullptr, nullptr);
+  const uint32_t G = tile_edges(tile);
+  hipLaunchKernelGGL(k_tri_enum<false>, dim3(tri_grid(e_end - e_begin, G)), dim3(kBlock), 0, s, O, E, src, e_begin,
+                     e_end, rows, inpos, nullptr, nullptr, nullptr, G);
   NUC34_LAUNCH_CHECK();
 }
 
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
                      const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
-                     hipStream_t s) {
+                     uint32_t tile, hipStream_t s) {
   if (e_end <= e_begin) return;
-  hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, inpos, triA, triBC, cIdx);
+  const uint32_t G = tile_edges(tile);
+  hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin, G)), dim3(kBlock), 0, s, O, E, src, e_begin,
+                     e_end, rows, inpos, triA, triBC, cIdx, G);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -173,6 +173,8 @@ typedef struct nuc34_tuning {
   uint32_t peel_tile_cap;          /* on-demand: most dying triangles one wave takes at a time, 1..64 (64) */
   uint32_t peel_rounds_per_launch; /* peel rounds per captured graph launch, even (32) */
   uint32_t k4_list_grid_cap;       /* most workgroups of the direct K4 list build (32768) */
+  uint32_t tri_tile_edges;         /* oriented edges per wave tile of the triangle intersection: 16, 32, 64 (64) */
+  uint32_t peel_finish_grid;       /* workgroups of a peel round's finish kernel (512) */
 } nuc34_tuning;
 
 int nuc34_set_tuning(nuc34_ctx* ctx, const nuc34_tuning* tuning);

@@ -313,7 +313,8 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
         assert ms > 0 and 1 <= t["peel_tile_cap"] <= 64 and t["peel_rounds_per_launch"] % 2 == 0
         for variant, shape in (("ondemand", {}), ("ondemand", dict(peel_debit_grid=96, peel_tile_cap=3,
                                                                     peel_rounds_per_launch=6)),
-                               ("direct", dict(peel_debit_grid=300, k4_list_grid_cap=17))):
+                               ("direct", dict(peel_debit_grid=300, k4_list_grid_cap=17)),
+                               ("direct", dict(tri_tile_edges=16)), ("ondemand", dict(tri_tile_edges=32))):
             if shape:
                 ctx.set_tuning(**shape)
             ctx.reset()
@@ -321,6 +322,8 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
             assert np.array_equal(ctx.copy_k(), k0) and st["rounds"] == st0["rounds"], (variant, shape)
         with pytest.raises(nuc34.Nuc34Error):
             ctx.set_tuning(peel_tile_cap=65)
+        with pytest.raises(nuc34.Nuc34Error):
+            ctx.set_tuning(tri_tile_edges=48)
 
 
 def level_peel_sim(tris, O, E):
@@ -394,3 +397,66 @@ def test_device_rates():
     print(f"[rates] {r}", flush=True)
     assert all(np.isfinite(v) and v > 0 for v in r.values())
     assert r["copy_gbps"] > 100 and r["atomic_gops"] > 0.5
+
+
+@pytest.mark.parametrize("variant", ["direct", "ondemand"])
+def test_peel_grids_beyond_residency(tmp_path, variant):
+    """Peel kernels with far more workgroups than the device holds at once
+    (late workgroups start after early ones have ended) give the same k and
+    rounds: no workgroup's control reads depend on another's writes."""
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 11, out=tmp_path / "g.edges")
+    ref = oracle34.OracleGraph.load(p, orient=True).run("ondemand", 8)
+    g = nuc34.load_dataset(p)
+    with nuc34.Context() as ctx:
+        ctx.load_graph(g)
+        for shape in (dict(peel_finish_grid=1 << 16), dict(peel_finish_grid=1 << 15, peel_debit_grid=1 << 15),
+                      dict(peel_finish_grid=1, peel_debit_grid=1)):
+            print(f"[grids] {variant} {shape}", flush=True)
+            ctx.set_tuning(**shape)
+            ctx.reset()
+            st = ctx.decompose(variant)
+            assert np.array_equal(ctx.copy_k(), ref["k"]), shape
+            assert st["rounds"] == ref["rounds"] and st["unpeeled"] == 0, shape
+
+
+def test_concurrent_contexts_bit_exact(tmp_path):
+    """Two contexts decomposing at the same time on their own streams (one
+    large graph over and over, many small ones beside it) give each graph the
+    result it gets alone."""
+    import threading
+    p = gen("rmat", 14, 16, 0.57, 0.19, 0.19, 5, out=tmp_path / "big.edges")
+    big = nuc34.load_dataset(p)
+    small = [nuc34.gen_batch_graph(4, i) for i in range(0, 40, 3)]
+    alone = {}
+    for i, g in enumerate(small):
+        for variant in ("direct", "ondemand"):
+            t, _, k, _ = gpu_run(g, variant)
+            alone[(i, variant)] = (t, k)
+    _, _, kbig, _ = gpu_run(big, "ondemand")
+    stop, errors = threading.Event(), []
+
+    def hog():
+        with nuc34.Context() as ctx:
+            ctx.load_graph(big)
+            while not stop.is_set():
+                ctx.reset()
+                ctx.decompose("ondemand")
+                if not np.array_equal(ctx.copy_k(), kbig):
+                    errors.append("large graph")
+                    return
+
+    th = threading.Thread(target=hog)
+    th.start()
+    try:
+        for rep in range(3):
+            print(f"[concurrent] pass {rep}", flush=True)
+            for i, g in enumerate(small):
+                for variant in ("direct", "ondemand"):
+                    t, _, k, _ = gpu_run(g, variant)
+                    et, ek = alone[(i, variant)]
+                    if not (np.array_equal(t, et) and np.array_equal(k, ek)):
+                        errors.append((rep, i, variant))
+    finally:
+        stop.set()
+        th.join()
+    assert not errors, errors
