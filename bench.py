@@ -122,16 +122,20 @@ def calibration_copy(mib: int) -> None:
             raise RuntimeError(f"HIP error {rc} in the calibration copy")
 
 
-def run_traffic(workload: str, group: str, launches: float, tmp: str, calib_mib: int = 1024) -> dict | None:
+def run_traffic(workload: str, group: str, launches: float, tmp: str, tuning: dict | None,
+                calib_mib: int = 1024) -> dict | None:
     """HBM bytes per launch of `group` from two rocprofv3 --pmc runs of a
     one-step bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
     by what the same counter reads for a device-to-device copy of `calib_mib` MiB
-    made first in the same process. None when rocprofv3 is unavailable or fails."""
+    made first in the same process. The child runs with the launch shapes of the
+    timed run (`tuning`), not a tuning pass of its own. None when rocprofv3 is
+    unavailable or fails."""
     prof = "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-batch",
-             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--workload", workload, "--copy-calibration-mib", str(calib_mib)]
+             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--workload", workload,
+             *(["--tuning", json.dumps(tuning)] if tuning else ["--no-autotune"]), "--copy-calibration-mib", str(calib_mib)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(tmp, counter)
@@ -314,6 +318,8 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="launch shapes as a JSON object of nuc34_tuning fields, used instead of autotuning")
     ap.add_argument("--no-rates", action="store_true",
                     help="skip measuring the device's copy bandwidth and random atomic / load rates")
     ap.add_argument("--serve-seconds", type=float, default=0.0,
@@ -340,7 +346,11 @@ def main():
         ctx.load_graph(g)
         O, E = ctx.oriented_csr()
         gs = graph_stats(O, E)
-        tuning = None if args.no_autotune else ctx.autotune(wl["variant"])[0]
+        if args.tuning:
+            ctx.set_tuning(**json.loads(args.tuning))
+            tuning = ctx.tuning()
+        else:
+            tuning = None if args.no_autotune else ctx.autotune(wl["variant"])[0]
         for _ in range(args.warmup):
             ctx.reset()
             ctx.decompose(wl["variant"], wl["parts"])
@@ -365,7 +375,7 @@ def main():
                 a["launches"] += k["launches"]
         dom = max(kern, key=lambda k: kern[k]["ms"])
         launches_per_step = kern[dom]["launches"] / args.steps
-        traffic = None if args.no_traffic else run_traffic(args.workload, dom, launches_per_step, tmp)
+        traffic = None if args.no_traffic else run_traffic(args.workload, dom, launches_per_step, tmp, tuning)
 
     ms = (t1 - t0) / args.steps * 1e3
     value = st["four_cliques"] / (ms / 1e3)
