@@ -22,6 +22,9 @@ Prints one JSON line with the driver's contract, plus:
   roofline.atomics  the peel's support decrements (device atomics) per second
                 against the measured random-atomic rate of the device;
   device_rates  measured copy GB/s and random 4-byte atomic / load rates;
+  config.kernel_gbs      algorithmic GB/s of every timed kernel group;
+  config.end_to_end_ms   parse, load (host to device, checks, orientation),
+                decomposition and the copy of k and the triangles back;
   cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
                 the reference headers) timed on this host's cores on a bounded
                 sample (R-MAT scale 14, same generator parameters).
@@ -170,6 +173,17 @@ def run_traffic(workload: str, group: str, launches: float, tmp: str, tuning: di
             "calibration": f"{calib_mib} MiB device-to-device copy in the profiled process"}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_cpu_baseline(tmp: str, threads: int) -> dict:
     ref = os.path.join(ROOT, "oracle", "_ref", "ref34")
     kind = "reference"
@@ -186,7 +200,8 @@ def run_cpu_baseline(tmp: str, threads: int) -> dict:
     out = subprocess.run(args, check=True, capture_output=True, text=True, env=env).stdout
     rec = json.loads(out.strip().splitlines()[-1])
     secs = sum(rec.get(k, 0.0) for k in ("tri_s", "tri_graph_s", "qud_s", "qud_graph_s", "peel_s"))
-    return {"value": rec["Q"] / secs, "unit": "4-cliques/s", "cores": threads, "kind": kind,
+    return {"value": rec["Q"] / secs, "unit": "4-cliques/s", "cores": threads, "kind": kind, "cpu": cpu_model(),
+            "phases_s": {k: rec.get(k, 0.0) for k in ("tri_s", "tri_graph_s", "qud_s", "qud_graph_s", "peel_s")},
             "sample": f"R-MAT scale 14, edge factor 16, same (a,b,c,d), seed 1: T={rec['T']}, Q={rec['Q']}, "
                       f"K_max={rec['K_max']}; decomposition {secs:.2f} s (load excluded); "
                       f"{'reference host path compiled from cliques.hpp+bucket_ex.hpp (serial pstl: no TBB)' if kind == 'reference' else 'oracle restatement'}"}
@@ -341,9 +356,13 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "graph.edges")
         subprocess.run([GEN, *wl["gen"], path], check=True, stdout=subprocess.DEVNULL)
+        t_parse = time.perf_counter()
         g = nuc34.load_dataset(path)
+        t_parse = time.perf_counter() - t_parse
         ctx = nuc34.Context(0)
-        ctx.load_graph(g)
+        t_load = time.perf_counter()
+        ctx.load_graph(g)  # host to device, checks, orientation on the device
+        t_load = time.perf_counter() - t_load
         O, E = ctx.oriented_csr()
         gs = graph_stats(O, E)
         if args.tuning:
@@ -360,6 +379,10 @@ def main():
             ctx.reset()
             steps.append(ctx.decompose(wl["variant"], wl["parts"]))
         t1 = time.perf_counter()
+        t_copy = time.perf_counter()
+        ctx.copy_k()
+        ctx.copy_triangles()
+        t_copy = time.perf_counter() - t_copy
         ctx.close()
         rates = None if args.no_rates else nuc34.device_rates(0)
         batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
@@ -379,6 +402,12 @@ def main():
 
     ms = (t1 - t0) / args.steps * 1e3
     value = st["four_cliques"] / (ms / 1e3)
+    end_to_end = {"parse_ms": t_parse * 1e3, "load_ms": t_load * 1e3, "decompose_ms": ms,
+                  "copy_k_and_triangles_ms": t_copy * 1e3}
+    end_to_end["total_ms"] = sum(end_to_end.values())
+    # achieved algorithmic GB/s of every timed kernel group (intersection, support, lists, peel)
+    kernel_gbs = {k: round(algorithmic_bytes(k, gs, st) / (v["ms"] / args.steps / 1e3) / 1e9, 1)
+                  for k, v in kern.items() if v["ms"] > 0 and algorithmic_bytes(k, gs, st) > 0}
     per_launch_ms = kern[dom]["ms"] / kern[dom]["launches"]
     bytes_per_launch = algorithmic_bytes(dom, gs, st) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
@@ -404,7 +433,8 @@ def main():
                    "counts_match_reference": parity, "tuning": tuning,
                    "phase_ms": {k: round(st[k], 3) for k in ("ms_triangles", "ms_support", "ms_k4_list",
                                                            "ms_incidence", "ms_peel", "ms_total")},
-                   "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()}},
+                   "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()},
+                   "kernel_gbs": kernel_gbs, "end_to_end_ms": {k: round(v, 3) for k, v in end_to_end.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

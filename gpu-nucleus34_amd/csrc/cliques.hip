@@ -49,7 +49,7 @@ struct TriRows {
   uint64_t rv0;  // Aout / Ain row of v's first out-edge
   uint64_t ca;   // Cm row of a (over N-(u)); b's is (jv - jw) * WC words further
   uint64_t cc;   // Cm row of c (over N-(v))
-  uint32_t c, ov, jv, jw;
+  uint32_t c, ou, ov, jv, jw;  // c-edge; O[u], O[v]; positions of v and w in N+(u)
   uint32_t W;    // words of the rows of u's out-edges
   uint32_t Wv;   // words of the rows of v's out-edges
   uint32_t WC;   // words of the a / b c-role rows
@@ -61,10 +61,10 @@ __device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
   const uint64_t bc = s.triBC[t];
   const uint32_t b = static_cast<uint32_t>(bc >> 32);
   r.c = static_cast<uint32_t>(bc);
-  const uint32_t ou = s.O[s.src[b]];
+  r.ou = s.O[s.src[b]];
   r.ov = s.O[s.E[b]];
-  r.jv = b - ou;
-  r.jw = a - ou;
+  r.jv = b - r.ou;
+  r.jw = a - r.ou;
   const uint64_t ra = s.rows.rowA[a];
   r.W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
   r.ru0 = ra - static_cast<uint64_t>(r.jw) * r.W;
@@ -92,6 +92,7 @@ __device__ __forceinline__ TriRows shfl_rows(const TriRows& r, uint32_t src) {
   o.ca = shfl_u64(r.ca, src);
   o.cc = shfl_u64(r.cc, src);
   o.c = shfl_u32(r.c, src);
+  o.ou = shfl_u32(r.ou, src);
   o.ov = shfl_u32(r.ov, src);
   o.jv = shfl_u32(r.jv, src);
   o.jw = shfl_u32(r.jw, src);
@@ -149,8 +150,11 @@ __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, ui
       q[2] = wx ? row_rank(Iv + static_cast<uint64_t>(evx - r.ov) * r.Wv, r.c - r.ov)  // v<w<x: a=(v,x), b=(v,w)
                 : row_rank(Iv + static_cast<uint64_t>(r.c - r.ov) * r.Wv, evx - r.ov);  // v<x<w: a=(v,w), b=(v,x)
     } else {
-      const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);  // x<v<w: a=(x,w), b=(x,v)
-      q[2] = row_rank(s.rows.Ain + s.rows.rowA[exw], evx - s.O[s.src[evx]]);
+      // x<v<w: a=(x,w), b=(x,v); x = N+(u)[jx] (consecutive for neighbouring
+      // lanes), v's position in N+(x) from the edge id of (x,v)
+      const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);
+      const uint32_t x = s.E[r.ou + jx];
+      q[2] = row_rank(s.rows.Ain + s.rows.rowA[exw], evx - s.O[x]);
     }
   } else {
     q[0] = s.cIdx[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
@@ -623,36 +627,31 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
 }
 
 // Prepares the slot this round's finish writes: its collection count and its
-// min reduction start empty. Only the debit kernel touches `out` (its finish
-// reads `out`'s fields only after it), and only after the kernel's last read
-// of its own control: a store through `out` earlier (it may alias `in` as far
-// as the compiler knows) turns every later read of the round's control into a
-// vector load.
-__device__ __forceinline__ void prepare_out_slot(PeelCtl* out) {
+// min reduction start empty, and a finished peel marks it done too, so that
+// rounds still queued in the captured launch do nothing whichever slot they
+// read. Only the debit kernel touches `out` (its finish reads `out`'s fields
+// only after it), and only at its very end, after the kernel's last read of
+// its own control: any store through `out` before that (it may alias `in` as
+// far as the compiler knows) turns the round's control reads into vector
+// loads.
+__device__ __forceinline__ void prepare_out_slot(PeelCtl* out, bool done) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     out->nspec = 0;
     out->minval = kNoMin;
+    if (done) out->done = 1;
   }
-}
-
-// A finished peel: rounds still queued in the captured launch do nothing, and
-// both slots end up marked done, whichever one the next round reads.
-__device__ __forceinline__ bool peel_done(const PeelCtl* in, PeelCtl* out) {
-  if (!in->done) return false;
-  if (blockIdx.x == 0 && threadIdx.x == 0) out->done = 1;
-  return true;
 }
 
 // Direct debit: one wave per dying triangle, one K4 list entry per lane.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
                                                              const uint64_t* __restrict__ koff,
                                                              const uint32_t* __restrict__ lists) {
-  if (peel_done(in, out)) return;
+  const bool done = in->done;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   uint32_t* ncount = &in->nnext;
-  const uint32_t n = in->ncur + in->nspec;
+  const uint32_t n = done ? 0u : in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint32_t nw = gridDim.x * kWPB;
@@ -666,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
       destroy_k4(t, qq, p, level, nxt, ncount);
     }
   }
-  prepare_out_slot(out);
+  prepare_out_slot(out, done);
 }
 
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
@@ -674,12 +673,12 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s,
                                                             uint32_t tile_cap) {
   __shared__ TileLds lds[kWPB];
-  if (peel_done(in, out)) return;
+  const bool done = in->done;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   uint32_t* ncount = &in->nnext;
-  const uint32_t n = in->ncur + in->nspec;
+  const uint32_t n = done ? 0u : in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
@@ -691,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
       destroy_k4(tt, q, p, level, nxt, ncount);
     });
   }
-  prepare_out_slot(out);
+  prepare_out_slot(out, done);
 }
 
 // Sum of the residual supports dd[0..T) into *total (which starts at zero).
