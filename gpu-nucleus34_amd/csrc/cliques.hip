@@ -290,28 +290,71 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
 
 // ---- support, K4 lists, canonical K4 list -------------------------------------
 
+// Support and owned counts, a wave per 64 triangles: the triangles' row words
+// (W up words of a and b, then WC down words) are flattened over the lanes 64
+// at a time, so a triangle whose lowest vertex has thousands of in-neighbours
+// (thousands of down words) is spread over many lanes instead of one; the
+// per-triangle sums gather in LDS.
 __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __restrict__ sup,
                                                          uint32_t* __restrict__ own, uint64_t* __restrict__ total) {
+  __shared__ uint32_t s_pre[kWPB][kWave + 1];
+  __shared__ uint32_t s_sup[kWPB][kWave];
+  __shared__ uint32_t s_own[kWPB][kWave];
+  const unsigned w = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
   uint64_t acc = 0;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < s.T; t += stride) {
-    const uint32_t a = s.triA[t];
-    const uint32_t b = static_cast<uint32_t>(s.triBC[t] >> 32);
-    const uint64_t ra = s.rows.rowA[a], rb = s.rows.rowA[b];
-    const uint32_t W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
-    uint32_t up = 0, ow = 0, down = 0;
-    for (uint32_t i = 0; i < W; ++i) {
-      const uint32_t oa = s.rows.Aout[ra + i], ob = s.rows.Aout[rb + i];
-      up += static_cast<uint32_t>(__popc((oa | s.rows.Ain[ra + i].bits) & (ob | s.rows.Ain[rb + i].bits)));
-      ow += static_cast<uint32_t>(__popc(oa & ob));
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < s.T; tile += nw) {
+    const uint64_t t = tile * kWave + lane;
+    const bool valid = t < s.T;
+    uint64_t ra = 0, rb = 0, ca = 0, cb = 0;
+    uint32_t W = 0, WC = 0;
+    if (valid) {
+      const uint32_t a = s.triA[t];
+      const uint32_t b = static_cast<uint32_t>(s.triBC[t] >> 32);
+      ra = s.rows.rowA[a];
+      rb = s.rows.rowA[b];
+      W = static_cast<uint32_t>(s.rows.rowA[a + 1] - ra);
+      ca = s.rows.rowC[a];
+      cb = s.rows.rowC[b];
+      WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - ca);  // a and b are out-edges of one vertex
     }
-    const uint64_t ca = s.rows.rowC[a], cb = s.rows.rowC[b];
-    const uint32_t WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - ca);
-    for (uint32_t i = 0; i < WC; ++i)
-      down += static_cast<uint32_t>(__popc(s.rows.Cm[ca + i].bits & s.rows.Cm[cb + i].bits));
-    sup[t] = static_cast<int32_t>(up + down);
-    own[t] = ow;
-    acc += up + down;
+    const uint32_t words = W + WC;
+    const uint32_t inc = wave_inclusive_sum(words);
+    const uint32_t nwords = __shfl(inc, kWave - 1);
+    s_pre[w][lane] = inc - words;
+    if (lane == kWave - 1) s_pre[w][kWave] = inc;
+    s_sup[w][lane] = 0;
+    s_own[w][lane] = 0;
+    wave_sync();
+    for (uint32_t base = 0; base < nwords; base += kWave) {
+      const uint32_t item = base + lane;
+      const bool act = item < nwords;
+      const uint32_t i = find_slot(s_pre[w], act ? item : nwords - 1);
+      // triangle i's rows, from lane i (every lane takes part)
+      const uint64_t rai = shfl_u64(ra, i), rbi = shfl_u64(rb, i), cai = shfl_u64(ca, i), cbi = shfl_u64(cb, i);
+      const uint32_t Wi = shfl_u32(W, i);
+      if (act) {
+        const uint32_t wi = item - s_pre[w][i];
+        uint32_t c = 0, o = 0;
+        if (wi < Wi) {
+          const uint32_t oa = s.rows.Aout[rai + wi], ob = s.rows.Aout[rbi + wi];
+          c = static_cast<uint32_t>(__popc((oa | s.rows.Ain[rai + wi].bits) & (ob | s.rows.Ain[rbi + wi].bits)));
+          o = static_cast<uint32_t>(__popc(oa & ob));
+        } else {
+          c = static_cast<uint32_t>(__popc(s.rows.Cm[cai + (wi - Wi)].bits & s.rows.Cm[cbi + (wi - Wi)].bits));
+        }
+        if (c) atomicAdd(&s_sup[w][i], c);
+        if (o) atomicAdd(&s_own[w][i], o);
+      }
+    }
+    wave_sync();
+    if (valid) {
+      sup[t] = static_cast<int32_t>(s_sup[w][lane]);
+      own[t] = s_own[w][lane];
+      acc += s_sup[w][lane];
+    }
+    wave_sync();
   }
   acc = wave_sum_u64(acc);
   if (lane_id() == 0 && acc) atomic_add_u64(total, acc);
@@ -729,7 +772,8 @@ void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s) {
 
 void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st) {
   if (!s.T) return;
-  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(s.T, kBlock, 1u << 14)), dim3(kBlock), 0, st, s, sup, own, total);
+  const uint64_t tiles = (s.T + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(tiles, kWPB, 1u << 14)), dim3(kBlock), 0, st, s, sup, own, total);
   NUC34_LAUNCH_CHECK();
 }
 
