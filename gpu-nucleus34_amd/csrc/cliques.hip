@@ -159,8 +159,9 @@ __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, ui
   } else {
     q[0] = s.cIdx[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
     q[1] = s.cIdx[rank_in_word(k.Ia, j)];  // T{x,u,w}: c-edge a
-    const uint32_t exv = s.triA[q[0]];     // a-edge of T{x,u,v}: (x,v)
-    q[2] = s.cIdx[row_rank(s.rows.Cm + r.cc, s.inpos[exv])];  // T{x,v,w}: c-edge c
+    // T{x,v,w}: c-edge c, at x's position in N-(v), the in-list position of
+    // (x,v), the a-edge of T{x,u,v}
+    q[2] = s.cIdx[row_rank(s.rows.Cm + r.cc, s.ainpos[q[0]])];
   }
 }
 

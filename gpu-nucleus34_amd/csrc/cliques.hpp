@@ -17,6 +17,7 @@ struct K4Src {
   const uint32_t* triA;   // a-edge per triangle
   const uint64_t* triBC;  // (b << 32) | c per triangle
   const uint32_t* cIdx;   // triangle id per c-slot
+  const uint32_t* ainpos; // per triangle: inpos of its a-edge
   uint64_t T;
   // Live rows for the on-demand peel (null elsewhere): LA = (Aout|Ain) and
   // LC = Cm bits, each triangle's three bits cleared after the round it dies
