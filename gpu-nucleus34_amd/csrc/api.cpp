@@ -410,18 +410,32 @@ void run_peel(nuc34_ctx* c, int variant) {
   c->st.ms_peel = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
     if (variant != NUC34_DIRECT) launch_live_rows(v, c->WA, c->WC, c->s);
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    struct Captured {  // destroyed on every way out, errors included
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      ~Captured() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+      }
+    } cap;
     if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {
-      enqueue_rounds(per_launch);
-      if (hipStreamEndCapture(c->s, &graph) != hipSuccess ||
-          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-        exec = nullptr;
+      try {
+        enqueue_rounds(per_launch);
+      } catch (...) {  // leave the stream out of capture mode before reporting
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(c->s, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+      }
+      if (hipStreamEndCapture(c->s, &cap.graph) != hipSuccess ||
+          hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0) != hipSuccess) {
+        cap.exec = nullptr;
         (void)hipGetLastError();
       }
     } else {
       (void)hipGetLastError();
     }
+    const hipGraphExec_t exec = cap.exec;
     const uint64_t max_launches = 2 * T / per_launch + 16;  // every second round fixes >= 1 triangle
     for (uint64_t l = 0;; ++l) {
       ktime(c, K_PEEL, per_launch, [&] {
@@ -436,8 +450,6 @@ void run_peel(nuc34_ctx* c, int variant) {
       if (h.done) break;
       if (l > max_launches) throw std::logic_error("peel did not finish");
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
   });
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
   c->st.rounds = h.round;

@@ -39,61 +39,80 @@ __global__ void k_random_load(const uint32_t* a, uint32_t n, uint64_t ops, uint3
 
 using namespace nuc34;
 
+namespace {
+
+// Buffers and events of one measurement, released on every way out.
+struct ProbeRes {
+  void* a = nullptr;
+  void* b = nullptr;
+  void* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~ProbeRes() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (sink) (void)hipFree(sink);
+  }
+};
+
+}  // namespace
+
 extern "C" int nuc34_measure_device(int device, nuc34_device_rates* out) {
   if (!out) return NUC34_E_INVALID;
   try {
     NUC34_HIP(hipSetDevice(device));
     const size_t bytes = size_t(1) << 30;
-    void *a = nullptr, *b = nullptr, *sink = nullptr;
-    NUC34_HIP(hipMalloc(&a, bytes));
-    NUC34_HIP(hipMalloc(&b, bytes));
-    NUC34_HIP(hipMalloc(&sink, 4));
-    hipEvent_t e0, e1;
-    NUC34_HIP(hipEventCreate(&e0));
-    NUC34_HIP(hipEventCreate(&e1));
-    NUC34_HIP(hipMemset(a, 0, bytes));
+    ProbeRes r;
+    for (void** p : {&r.a, &r.b}) {
+      const hipError_t e = hipMalloc(p, bytes);
+      if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        return NUC34_E_OOM;
+      }
+      NUC34_HIP(e);
+    }
+    NUC34_HIP(hipMalloc(&r.sink, 4));
+    NUC34_HIP(hipEventCreate(&r.e0));
+    NUC34_HIP(hipEventCreate(&r.e1));
+    NUC34_HIP(hipMemset(r.a, 0, bytes));
     auto best_ms = [&](auto&& launch) {
       float best = 1e30f;
-      for (int r = 0; r < 3; ++r) {
-        NUC34_HIP(hipEventRecord(e0, nullptr));
+      for (int rep = 0; rep < 3; ++rep) {
+        NUC34_HIP(hipEventRecord(r.e0, nullptr));
         launch();
-        NUC34_HIP(hipEventRecord(e1, nullptr));
-        NUC34_HIP(hipEventSynchronize(e1));
+        NUC34_HIP(hipEventRecord(r.e1, nullptr));
+        NUC34_HIP(hipEventSynchronize(r.e1));
         float ms = 0;
-        NUC34_HIP(hipEventElapsedTime(&ms, e0, e1));
+        NUC34_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
         best = std::min(best, ms);
       }
       return static_cast<double>(best);
     };
-    const double copy_ms = best_ms([&] { NUC34_HIP(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr)); });
+    const double copy_ms =
+        best_ms([&] { NUC34_HIP(hipMemcpyAsync(r.b, r.a, bytes, hipMemcpyDeviceToDevice, nullptr)); });
     const uint64_t ops = uint64_t(1) << 28;
     const uint32_t n_atomic = 16u << 20, n_l2 = 1u << 20, n_hbm = 64u << 20;  // 64 MiB, 4 MiB, 256 MiB
     const double atomic_ms = best_ms([&] {
-      hipLaunchKernelGGL(k_random_atomic, dim3(8192), dim3(256), 0, nullptr, static_cast<int32_t*>(a), n_atomic, ops);
+      hipLaunchKernelGGL(k_random_atomic, dim3(8192), dim3(256), 0, nullptr, static_cast<int32_t*>(r.a), n_atomic,
+                         ops);
       NUC34_LAUNCH_CHECK();
     });
     const double l2_ms = best_ms([&] {
-      hipLaunchKernelGGL(k_random_load, dim3(8192), dim3(256), 0, nullptr, static_cast<const uint32_t*>(a), n_l2, ops,
-                         static_cast<uint32_t*>(sink));
+      hipLaunchKernelGGL(k_random_load, dim3(8192), dim3(256), 0, nullptr, static_cast<const uint32_t*>(r.a), n_l2,
+                         ops, static_cast<uint32_t*>(r.sink));
       NUC34_LAUNCH_CHECK();
     });
     const double hbm_ms = best_ms([&] {
-      hipLaunchKernelGGL(k_random_load, dim3(8192), dim3(256), 0, nullptr, static_cast<const uint32_t*>(a), n_hbm, ops,
-                         static_cast<uint32_t*>(sink));
+      hipLaunchKernelGGL(k_random_load, dim3(8192), dim3(256), 0, nullptr, static_cast<const uint32_t*>(r.a), n_hbm,
+                         ops, static_cast<uint32_t*>(r.sink));
       NUC34_LAUNCH_CHECK();
     });
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
-    (void)hipFree(sink);
     out->copy_gbps = 2.0 * static_cast<double>(bytes) / (copy_ms * 1e6);
     out->atomic_gops = static_cast<double>(ops) / (atomic_ms * 1e6);
     out->load_l2_gops = static_cast<double>(ops) / (l2_ms * 1e6);
     out->load_hbm_gops = static_cast<double>(ops) / (hbm_ms * 1e6);
     return NUC34_OK;
-  } catch (const OutOfMemory&) {
-    return NUC34_E_OOM;
   } catch (const std::exception&) {
     return NUC34_E_HIP;
   }

@@ -9,8 +9,9 @@
 // decide between one-shot and segmented work from free memory. Here each lane
 // holds a share of the HBM budget; a batch runs in the configured variant (by
 // default on demand, O(T) memory, or direct with 48 bytes per K4 once the
-// support count has given Q), and a batch that exceeds the lane's budget is
-// split in halves and retried, down to single jobs.
+// support count has given Q), and a batch that exceeds the lane's budget (or
+// overflows, or holds a malformed graph) is split in halves and retried, down
+// to single jobs.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -179,7 +180,11 @@ void nuc34_server::run_batch(Worker* w, std::vector<Job*>& batch) {
   }
   const double t_start = now();
   const int rc = decompose_into(w, batch, bid, t_start);
-  if (rc == NUC34_E_OOM && batch.size() > 1) {
+  // a batch that does not fit, whose union overflows the 32-bit ids, or that
+  // holds a malformed graph is split in halves and retried, so the failure
+  // ends up with the job that causes it and its batch-mates still run
+  const bool splittable = rc == NUC34_E_OOM || rc == NUC34_E_OVERFLOW || rc == NUC34_E_INVALID;
+  if (splittable && batch.size() > 1) {
     {
       std::lock_guard<std::mutex> lk(mu);
       st.splits++;

@@ -94,3 +94,19 @@ def test_serve_empty_and_triangle_free():
             tris, k = srv.take(j, 0)
             assert len(tris) == 0 and len(k) == 0
         assert srv.wait(j0, 0) is None  # taken jobs are forgotten
+
+
+def test_serve_malformed_job_fails_alone():
+    """A malformed graph batched with valid ones fails with NUC34_E_INVALID and
+    the others still come back exact."""
+    print("[serve] one malformed job in a batch of valid ones", flush=True)
+    gs = [nuc34.gen_batch_graph(4, i) for i in range(1, 12, 2)]
+    bad = (np.array([0, 2, 3, 4], np.uint32), np.array([2, 1, 0, 0], np.uint32))  # vertex 0's list unsorted
+    with nuc34.Server(hbm_budget=8 << 30, batch_window_ms=1000) as srv:
+        jobs = [srv.submit(g) for g in gs[:3]] + [srv.submit(bad)] + [srv.submit(g) for g in gs[3:]]
+        info = srv.wait(jobs[3], 60000)
+        assert info is not None and info["status"] == nuc34.E_INVALID, info
+        good = jobs[:3] + jobs[4:]
+        check_jobs(srv, good, gs)
+        st = srv.stats()
+    assert st["failed"] == 1 and st["completed"] == len(gs) and st["splits"] >= 1
