@@ -597,6 +597,7 @@ int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
 }
 
 int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
+  if (!out) return NUC34_E_INVALID;
   auto* c = new nuc34_ctx;
   c->device = device;
   c->alloc.budget = hbm_budget;
@@ -607,7 +608,7 @@ int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
     return NUC34_OK;
   });
   if (rc != NUC34_OK) {
-    delete c;
+    nuc34_ctx_destroy(c);  // releases whatever was created
     *out = nullptr;
     return rc;
   }

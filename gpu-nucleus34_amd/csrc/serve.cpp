@@ -257,6 +257,7 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
   if (!c.max_batch_edges) c.max_batch_edges = 256ull << 20;
   if (!c.max_batch_graphs) c.max_batch_graphs = 4096;
   if (!c.large_workers) c.large_workers = 2;
+  if (c.large_workers > 64) return NUC34_E_INVALID;  // each worker holds a context, a stream and a thread
   if (!c.hbm_budget) {
     char buf[1024];
     if (nuc34_device_info(c.device, buf, sizeof(buf)) != NUC34_OK) return NUC34_E_HIP;
