@@ -229,7 +229,7 @@ typedef struct nuc34_serve_config {
   int variant;               /* NUC34_AUTO (default), NUC34_DIRECT or NUC34_ONDEMAND */
   double batch_window_ms;    /* small-job lane: wait this long after its oldest queued job
                                 arrived before starting a batch (0 = start at once) */
-  uint32_t large_workers;    /* workers (contexts, streams) for large jobs (0 = 2) */
+  uint32_t large_workers;    /* workers (contexts, streams) for large jobs (0 = 2, at most 64) */
 } nuc34_serve_config;
 
 typedef struct nuc34_job_info {
