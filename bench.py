@@ -23,6 +23,8 @@ Prints one JSON line with the driver's contract, plus:
                 against the measured random-atomic rate of the device;
   device_rates  measured copy GB/s and random 4-byte atomic / load rates;
   config.kernel_gbs      algorithmic GB/s of every timed kernel group;
+  other_variant the other variant (direct <-> on demand) on the same graph, with
+                default launch shapes, and whether its k equals the headline's;
   config.end_to_end_ms   parse, load (host to device, checks, orientation),
                 decomposition and the copy of k and the triangles back;
   cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
@@ -137,7 +139,7 @@ def run_traffic(workload: str, group: str, launches: float, tmp: str, tuning: di
     if not os.path.exists(prof):
         return None
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-batch",
-             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--workload", workload,
+             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--no-other-variant", "--workload", workload,
              *(["--tuning", json.dumps(tuning)] if tuning else ["--no-autotune"]), "--copy-calibration-mib", str(calib_mib)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -335,6 +337,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tuning", default=None,
                     help="launch shapes as a JSON object of nuc34_tuning fields, used instead of autotuning")
+    ap.add_argument("--no-other-variant", action="store_true",
+                    help="skip timing the other variant (direct <-> on demand) on the same graph")
     ap.add_argument("--no-rates", action="store_true",
                     help="skip measuring the device's copy bandwidth and random atomic / load rates")
     ap.add_argument("--serve-seconds", type=float, default=0.0,
@@ -380,9 +384,29 @@ def main():
             steps.append(ctx.decompose(wl["variant"], wl["parts"]))
         t1 = time.perf_counter()
         t_copy = time.perf_counter()
-        ctx.copy_k()
+        k_main = ctx.copy_k()
         ctx.copy_triangles()
         t_copy = time.perf_counter() - t_copy
+        other = None
+        if not args.no_other_variant:
+            # the other variant on the same graph (default launch shapes), timed the
+            # same way; its k must equal the headline variant's
+            ov = "ondemand" if wl["variant"] == "direct" else "direct"
+            if tuning:
+                ctx.set_tuning(**{f: 0 for f in tuning})
+            try:
+                ctx.reset()
+                ctx.decompose(ov, wl["parts"])
+                t2 = time.perf_counter()
+                for _ in range(args.steps):
+                    ctx.reset()
+                    so = ctx.decompose(ov, wl["parts"])
+                t3 = time.perf_counter()
+                ms_o = (t3 - t2) / args.steps * 1e3
+                other = {"variant": ov, "ms_per_step": ms_o, "value": so["four_cliques"] / (ms_o / 1e3),
+                         "k_equal": bool(np.array_equal(ctx.copy_k(), k_main))}
+            except nuc34.Nuc34Error as e:  # e.g. the direct lists do not fit
+                other = {"variant": ov, "error": str(e)}
         ctx.close()
         rates = None if args.no_rates else nuc34.device_rates(0)
         batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
@@ -442,6 +466,7 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
                      "atomics": peel_atomics(st, rates)},
         "device_rates": rates,
+        "other_variant": other,
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
