@@ -907,7 +907,7 @@ int nuc34_copy_support(nuc34_ctx* c, int32_t* support) {
 
 int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stats) {
   if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
-  if (variant != NUC34_DIRECT && variant != NUC34_ONDEMAND && variant != NUC34_AUTO)
+  if (variant != NUC34_DIRECT && variant != NUC34_ONDEMAND && variant != NUC34_AUTO && variant != NUC34_AUTO_FIT)
     return fail(c, NUC34_E_INVALID, "unknown variant");
   return guard(c, [&]() -> int {
     if (!c->has_tris) {
@@ -925,7 +925,21 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
     // it needs no 48 bytes per K4 and ran faster than the direct variant on
     // every measured workload (one graph and many, 10^4 to 10^10 K4s); direct
     // runs only when asked for.
-    const int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
+    int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
+    if (variant == NUC34_AUTO_FIT) {
+      // the direct variant's extra memory: the lists (48 bytes per K4) and
+      // their offsets, plus the peel's O(T) state both variants hold
+      const uint64_t need = 48 * c->Q + 8 * (c->T + 1) + 20 * c->T;
+      uint64_t room = 0;
+      if (c->alloc.budget) {
+        room = c->alloc.budget > c->alloc.used ? c->alloc.budget - c->alloc.used : 0;
+      } else {
+        size_t free_b = 0, total_b = 0;
+        NUC34_HIP(hipMemGetInfo(&free_b, &total_b));
+        room = free_b + c->alloc.cached;
+      }
+      v = need + need / 4 <= room ? NUC34_DIRECT : NUC34_ONDEMAND;
+    }
     reset_stages(c, 4);
     // the on-demand variant needs nothing beyond the bit rows: each dying
     // triangle's K4s are enumerated from them in the peel

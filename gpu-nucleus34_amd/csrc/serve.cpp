@@ -251,7 +251,7 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
   auto s = std::make_unique<nuc34_server>();
   if (cfg) s->cfg = *cfg;
   nuc34_serve_config& c = s->cfg;
-  if (c.variant != NUC34_DIRECT && c.variant != NUC34_ONDEMAND) c.variant = NUC34_AUTO;
+  if (c.variant != NUC34_DIRECT && c.variant != NUC34_ONDEMAND && c.variant != NUC34_AUTO_FIT) c.variant = NUC34_AUTO;
   if (c.large_fraction <= 0 || c.large_fraction >= 1) c.large_fraction = 0.75;
   if (!c.large_edges) c.large_edges = 4ull << 20;
   if (!c.max_batch_edges) c.max_batch_edges = 256ull << 20;

@@ -19,8 +19,8 @@ BUILD_DIR = os.path.join(os.path.dirname(_HERE), "_build")
 LIB_PATH = os.path.join(BUILD_DIR, "libnuc34.so")
 
 OK, E_OOM, E_OVERFLOW, E_HIP, E_INVALID, E_IO, E_STATE = range(7)
-DIRECT, ONDEMAND, AUTO = 0, 1, 2
-VARIANTS = {"direct": DIRECT, "ondemand": ONDEMAND, "auto": AUTO}
+DIRECT, ONDEMAND, AUTO, AUTO_FIT = 0, 1, 2, 3
+VARIANTS = {"direct": DIRECT, "ondemand": ONDEMAND, "auto": AUTO, "auto_fit": AUTO_FIT}
 
 
 class Nuc34Error(RuntimeError):

@@ -44,7 +44,7 @@ enum {
   NUC34_E_STATE = 6     /* call out of order (e.g. no graph loaded) */
 };
 
-enum { NUC34_DIRECT = 0, NUC34_ONDEMAND = 1, NUC34_AUTO = 2 };
+enum { NUC34_DIRECT = 0, NUC34_ONDEMAND = 1, NUC34_AUTO = 2, NUC34_AUTO_FIT = 3 };
 
 typedef struct nuc34_ctx nuc34_ctx;
 typedef struct nuc34_graph nuc34_graph;
@@ -153,8 +153,12 @@ int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
 
 /* Runs the missing stages and the peel. variant: NUC34_DIRECT (per-triangle
  * K4 lists in HBM, 48 bytes per K4), NUC34_ONDEMAND (K4s re-enumerated from the
- * bit-packed incidence as triangles die, O(T) memory) or NUC34_AUTO (on-demand:
- * never slower and never larger on the measured workloads). */
+ * bit-packed incidence as triangles die, O(T) memory), NUC34_AUTO (on-demand:
+ * never slower and never larger on the measured workloads) or NUC34_AUTO_FIT
+ * (direct when, with Q known from the support count, its lists fit what is
+ * left of the context's budget, or of free device memory without one, with a
+ * quarter to spare; on demand otherwise: the reference's own choice between
+ * one-shot and segmented work from free memory, nucleus/utils.cuh:1329-1340). */
 int nuc34_decompose(nuc34_ctx* ctx, int variant, uint32_t parts, nuc34_stats* stats);
 /* k for every triangle in (a,b) order (0 for triangles in no K4). */
 int nuc34_copy_k(nuc34_ctx* ctx, int32_t* k);
@@ -226,7 +230,7 @@ typedef struct nuc34_serve_config {
   uint64_t large_edges;      /* undirected edge entries (2 per edge) at which a job is large (0 = 4 Mi) */
   uint64_t max_batch_edges;  /* edge entries per small-job batch (0 = 256 Mi) */
   uint32_t max_batch_graphs; /* graphs per small-job batch (0 = 4096) */
-  int variant;               /* NUC34_AUTO (default), NUC34_DIRECT or NUC34_ONDEMAND */
+  int variant;               /* NUC34_AUTO (default), NUC34_AUTO_FIT, NUC34_DIRECT or NUC34_ONDEMAND */
   double batch_window_ms;    /* small-job lane: wait this long after its oldest queued job
                                 arrived before starting a batch (0 = start at once) */
   uint32_t large_workers;    /* workers (contexts, streams) for large jobs (0 = 2, at most 64) */

@@ -460,3 +460,20 @@ def test_concurrent_contexts_bit_exact(tmp_path):
         stop.set()
         th.join()
     assert not errors, errors
+
+
+def test_auto_fit_picks_by_footprint(tmp_path):
+    """NUC34_AUTO_FIT runs direct when the K4 lists fit what is left of the
+    budget and on demand when they do not; k is the same either way."""
+    p = gen("rmat", 12, 16, 0.57, 0.19, 0.19, 10, out=tmp_path / "g.edges")
+    g = nuc34.load_dataset(p)
+    with nuc34.Context(0) as ctx:  # no budget: the device's free memory decides
+        ctx.load_graph(g)
+        st = ctx.decompose("auto_fit")
+        assert st["variant"] == nuc34.DIRECT
+        k_direct = ctx.copy_k()
+    with nuc34.Context(0, 64 << 20) as ctx:  # O(T) state fits, the lists do not
+        ctx.load_graph(g)
+        st = ctx.decompose("auto_fit")
+        assert st["variant"] == nuc34.ONDEMAND
+        assert np.array_equal(ctx.copy_k(), k_direct)

@@ -110,3 +110,15 @@ def test_serve_malformed_job_fails_alone():
         check_jobs(srv, good, gs)
         st = srv.stats()
     assert st["failed"] == 1 and st["completed"] == len(gs) and st["splits"] >= 1
+
+
+def test_serve_auto_fit_variant_per_batch():
+    """With NUC34_AUTO_FIT each batch picks direct or on demand from its K4
+    count and the lane's budget; results are exact either way."""
+    print("[serve] auto_fit: small budget lane and large budget lane", flush=True)
+    gs = [nuc34.gen_batch_graph(4, i) for i in range(0, 16, 3)]
+    with nuc34.Server(hbm_budget=8 << 30, variant="auto_fit") as srv:
+        jobs = [srv.submit(g) for g in gs]
+        infos = check_jobs(srv, jobs, gs)
+    assert {i["variant"] for i in infos} <= {nuc34.DIRECT, nuc34.ONDEMAND}
+    assert any(i["variant"] == nuc34.DIRECT for i in infos)
