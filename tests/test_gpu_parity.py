@@ -477,3 +477,55 @@ def test_auto_fit_picks_by_footprint(tmp_path):
         st = ctx.decompose("auto_fit")
         assert st["variant"] == nuc34.ONDEMAND
         assert np.array_equal(ctx.copy_k(), k_direct)
+
+
+def random_graph(rng, kind):
+    """Small graphs of assorted shapes: sparse and dense ER, overlapping planted
+    cliques, a hub with a dense neighbourhood, ids spread over a wide range."""
+    if kind == "er":
+        n = int(rng.integers(20, 300))
+        p = float(rng.uniform(0.02, 0.4))
+        iu = np.triu_indices(n, 1)
+        keep = rng.random(len(iu[0])) < p
+        a, b = iu[0][keep], iu[1][keep]
+    elif kind == "cliques":
+        n = int(rng.integers(30, 200))
+        pairs = []
+        for _ in range(int(rng.integers(2, 8))):
+            vs = rng.choice(n, size=int(rng.integers(4, 14)), replace=False)
+            pairs += [(x, y) for i, x in enumerate(vs) for y in vs[i + 1:]]
+        pairs += [tuple(rng.choice(n, 2, replace=False)) for _ in range(n)]
+        a, b = np.array(pairs).T
+    elif kind == "hub":
+        n = int(rng.integers(50, 250))
+        hub_nbrs = rng.choice(np.arange(1, n), size=n // 2, replace=False)
+        pairs = [(0, int(x)) for x in hub_nbrs]
+        pairs += [tuple(rng.choice(hub_nbrs, 2, replace=False)) for _ in range(3 * n)]
+        a, b = np.array(pairs).T
+    else:  # spread ids: the same edges on ids scattered up to 50x the vertex count
+        n = int(rng.integers(20, 120))
+        iu = np.triu_indices(n, 1)
+        keep = rng.random(len(iu[0])) < 0.3
+        ids = np.sort(rng.choice(50 * n, size=n, replace=False))
+        a, b = ids[iu[0][keep]], ids[iu[1][keep]]
+    a, b = np.asarray(a, np.uint32), np.asarray(b, np.uint32)
+    m = a != b
+    return nuc34.graph_from_pairs(a[m], b[m])
+
+
+@pytest.mark.parametrize("kind", ["er", "cliques", "hub", "spread"])
+def test_random_small_graphs_vs_oracle(kind):
+    """Many small random graphs per shape: triangles, support, k, K_max and
+    rounds equal the oracle's in both variants and with parts > 1."""
+    rng = np.random.default_rng({"er": 11, "cliques": 12, "hub": 13, "spread": 14}[kind])
+    for it in range(12):
+        g = random_graph(rng, kind)
+        print(f"[random] {kind} #{it} n={g.n} m={g.m}", flush=True)
+        ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 4)
+        for variant, parts in (("direct", 1), ("ondemand", 1), ("direct", 3)):
+            tris, sup, k, st = gpu_run(g, variant, parts)
+            assert st["triangles"] == ref["T"] and st["four_cliques"] == ref["Q"], (kind, it, variant)
+            assert np.array_equal(tris, ref["tris"]) and np.array_equal(sup, ref["support"]), (kind, it, variant)
+            assert np.array_equal(k, ref["k"]), (kind, it, variant, parts)
+            if ref["Q"]:
+                assert st["k_max"] == ref["K_max"] and st["rounds"] == ref["rounds"], (kind, it, variant)
