@@ -713,35 +713,52 @@ int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t 
   });
 }
 
-int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
-                     const uint32_t* E, int oriented) {
-  if (!c || !n || !m || !O || (count && !E)) return fail(c, NUC34_E_INVALID, "null argument");
-  if (count == 1) {  // the union of one graph is the graph
-    const int rc = nuc34_load_csr(c, O, E, n[0], m[0], oriented);
-    if (rc != NUC34_OK) return rc;
-    c->batch_v = {0, n[0]};
-    c->batch_e = {0, c->m};
-    return NUC34_OK;
-  }
+}  // extern "C"
+
+namespace {
+
+// The graphs of a batch: back to back in two arrays, or one pointer pair per graph.
+struct BatchSrc {
+  const uint32_t* O = nullptr;
+  const uint32_t* E = nullptr;
+  const uint32_t* const* Op = nullptr;
+  const uint32_t* const* Ep = nullptr;
+  const uint32_t* og(uint32_t g, const std::vector<uint64_t>& vfirst) const { return Op ? Op[g] : O + vfirst[g] + g; }
+  const uint32_t* eg(uint32_t g, const std::vector<uint64_t>& efirst) const { return Ep ? Ep[g] : E + efirst[g]; }
+};
+
+int load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m, const BatchSrc& src, int oriented) {
   std::vector<uint64_t> vfirst(count + 1, 0), efirst(count + 1, 0);
   for (uint32_t g = 0; g < count; ++g) {
     vfirst[g + 1] = vfirst[g] + n[g];
     efirst[g + 1] = efirst[g] + m[g];
   }
+  if (src.Op)
+    for (uint32_t g = 0; g < count; ++g)
+      if (!src.Op[g] || (m[g] && !src.Ep[g])) return fail(c, NUC34_E_INVALID, "null graph " + std::to_string(g));
+  if (count == 1) {  // the union of one graph is the graph
+    const uint32_t* O0 = src.og(0, vfirst);
+    const int rc = nuc34_load_csr(c, O0, m[0] ? src.eg(0, efirst) : O0, n[0], m[0], oriented);
+    if (rc != NUC34_OK) return rc;
+    c->batch_v = {0, n[0]};
+    c->batch_e = {0, c->m};
+    return NUC34_OK;
+  }
   const uint64_t nv = vfirst[count], ne = efirst[count];
   if (nv >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 vertices");
   if (ne >= 0xFFFFFFFFull) return fail(c, NUC34_E_OVERFLOW, "batch has more than 2^32-1 edges");
   for (uint32_t g = 0; g < count; ++g) {
-    const uint32_t* Og = O + vfirst[g] + g;
+    const uint32_t* Og = src.og(g, vfirst);
     if (Og[0] != 0 || Og[n[g]] != m[g])
       return fail(c, NUC34_E_INVALID, "graph " + std::to_string(g) + ": offsets do not span its edges");
   }
   if (oriented) {  // rare: build the union on the host, where the acyclicity check runs
     std::vector<uint32_t> Ou(nv + 1), Eu(ne);
     for (uint32_t g = 0; g < count; ++g) {
-      const uint32_t* Og = O + vfirst[g] + g;
+      const uint32_t* Og = src.og(g, vfirst);
+      const uint32_t* Eg = src.eg(g, efirst);
       for (uint32_t v = 0; v < n[g]; ++v) Ou[vfirst[g] + v] = static_cast<uint32_t>(efirst[g] + Og[v]);
-      for (uint64_t i = 0; i < m[g]; ++i) Eu[efirst[g] + i] = E[efirst[g] + i] + static_cast<uint32_t>(vfirst[g]);
+      for (uint64_t i = 0; i < m[g]; ++i) Eu[efirst[g] + i] = Eg[i] + static_cast<uint32_t>(vfirst[g]);
     }
     Ou[nv] = static_cast<uint32_t>(ne);
     const int rc = nuc34_load_csr(c, Ou.data(), Eu.empty() ? Ou.data() : Eu.data(), static_cast<uint32_t>(nv), ne, 1);
@@ -755,8 +772,17 @@ int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint
       DevBuf Oin = c->alloc.get((nv + count) * 4, "batch offsets");
       DevBuf Ein = c->alloc.get(ne * 4 + 4, "batch edges");
       DevBuf firsts = c->alloc.get((count + 1) * 16, "batch boundaries");
-      NUC34_HIP(hipMemcpyAsync(Oin.as<void>(), O, (nv + count) * 4, hipMemcpyHostToDevice, c->s));
-      if (ne) NUC34_HIP(hipMemcpyAsync(Ein.as<void>(), E, ne * 4, hipMemcpyHostToDevice, c->s));
+      if (!src.Op) {
+        NUC34_HIP(hipMemcpyAsync(Oin.as<void>(), src.O, (nv + count) * 4, hipMemcpyHostToDevice, c->s));
+        if (ne) NUC34_HIP(hipMemcpyAsync(Ein.as<void>(), src.E, ne * 4, hipMemcpyHostToDevice, c->s));
+      } else {  // each graph straight into its place: no host-side concatenation
+        for (uint32_t g = 0; g < count; ++g) {
+          NUC34_HIP(hipMemcpyAsync(Oin.as<uint32_t>() + vfirst[g] + g, src.Op[g], (n[g] + 1ull) * 4,
+                                   hipMemcpyHostToDevice, c->s));
+          if (m[g])
+            NUC34_HIP(hipMemcpyAsync(Ein.as<uint32_t>() + efirst[g], src.Ep[g], m[g] * 4, hipMemcpyHostToDevice, c->s));
+        }
+      }
       NUC34_HIP(hipMemcpyAsync(firsts.as<void>(), vfirst.data(), (count + 1) * 8, hipMemcpyHostToDevice, c->s));
       NUC34_HIP(hipMemcpyAsync(firsts.as<uint64_t>() + count + 1, efirst.data(), (count + 1) * 8,
                                hipMemcpyHostToDevice, c->s));
@@ -792,6 +818,28 @@ int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint
     c->batch_v = vfirst;
     return NUC34_OK;
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuc34_load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
+                     const uint32_t* E, int oriented) {
+  if (!c || !n || !m || !O || (count && !E)) return fail(c, NUC34_E_INVALID, "null argument");
+  BatchSrc src;
+  src.O = O;
+  src.E = E;
+  return load_batch(c, count, n, m, src, oriented);
+}
+
+int nuc34_load_batch_ptrs(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* m,
+                          const uint32_t* const* O, const uint32_t* const* E, int oriented) {
+  if (!c || !n || !m || !O || (count && !E)) return fail(c, NUC34_E_INVALID, "null argument");
+  BatchSrc src;
+  src.Op = O;
+  src.Ep = E;
+  return load_batch(c, count, n, m, src, oriented);
 }
 
 int nuc34_batch_graph(nuc34_ctx* c, uint32_t g, uint64_t* vertex_first, uint64_t* edge_first, uint64_t* tri_first,

@@ -80,28 +80,17 @@ struct nuc34_server {
 int nuc34_server::decompose_into(Worker* w, std::vector<Job*>& batch, uint64_t batch_id, double t_start) {
   nuc34_ctx* ctx = w->ctx;
   const uint32_t count = static_cast<uint32_t>(batch.size());
-  std::vector<uint32_t> ns(count), Ocat, Ecat;
+  std::vector<uint32_t> ns(count);
   std::vector<uint64_t> ms(count);
-  uint64_t tv = 0, te = 0;
+  std::vector<const uint32_t*> Op(count), Ep(count);
   for (uint32_t g = 0; g < count; ++g) {
     ns[g] = batch[g]->n;
     ms[g] = batch[g]->m;
-    tv += batch[g]->n + 1;
-    te += batch[g]->m;
+    Op[g] = batch[g]->O.data();
+    Ep[g] = batch[g]->m ? batch[g]->E.data() : nullptr;
   }
-  const uint32_t *O = batch[0]->O.data(), *E = batch[0]->E.data();
-  if (count > 1) {
-    Ocat.reserve(tv);
-    Ecat.reserve(te);
-    for (Job* j : batch) {
-      Ocat.insert(Ocat.end(), j->O.begin(), j->O.end());
-      Ecat.insert(Ecat.end(), j->E.begin(), j->E.end());
-    }
-    O = Ocat.data();
-    E = Ecat.data();
-  }
-  // with no edges at all E is never read; any non-null pointer will do
-  int rc = nuc34_load_batch(ctx, count, ns.data(), ms.data(), O, te ? E : O, 0);
+  // each job's arrays go straight into the union on the device
+  int rc = nuc34_load_batch_ptrs(ctx, count, ns.data(), ms.data(), Op.data(), Ep.data(), 0);
   nuc34_stats s{};
   if (rc == NUC34_OK) rc = nuc34_decompose(ctx, cfg.variant, 1, &s);
   if (rc == NUC34_OK && count == 1) {  // one job: results straight into it

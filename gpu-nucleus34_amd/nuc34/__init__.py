@@ -121,6 +121,7 @@ def _load() -> C.CDLL:
         "nuc34_get_stats": (C.c_int, [vp, C.POINTER(Stats)]),
         "nuc34_reset": (C.c_int, [vp]),
         "nuc34_load_batch": (C.c_int, [vp, C.c_uint32, u32p, u64p, u32p, u32p, C.c_int]),
+        "nuc34_load_batch_ptrs": (C.c_int, [vp, C.c_uint32, u32p, u64p, C.POINTER(u32p), C.POINTER(u32p), C.c_int]),
         "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
         "nuc34_batch_slices": (C.c_int, [vp, u64p, u64p]),
         "nuc34_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -294,12 +295,19 @@ class Context:
             Es.append(np.ascontiguousarray(E, dtype=np.uint32))
             ns.append(len(O) - 1)
             ms.append(len(E))
-        O = np.concatenate(Os) if Os else np.zeros(1, np.uint32)
-        E = np.concatenate(Es) if Es and sum(ms) else np.zeros(1, np.uint32)
         n = np.array(ns, dtype=np.uint32)
         m = np.array(ms, dtype=np.uint64)
-        self._check(_lib.nuc34_load_batch(self._h, len(ns), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                          _u32(O), _u32(E), int(bool(oriented))))
+        if not ns:
+            z = np.zeros(1, np.uint32)
+            self._check(_lib.nuc34_load_batch(self._h, 0, _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              _u32(z), _u32(z), int(bool(oriented))))
+        else:
+            # one pointer per graph: the library uploads each into place (the
+            # arrays in Os/Es stay alive for the call)
+            Op = (C.POINTER(C.c_uint32) * len(ns))(*[_u32(a) for a in Os])
+            Ep = (C.POINTER(C.c_uint32) * len(ns))(*[_u32(a) if len(a) else None for a in Es])
+            self._check(_lib.nuc34_load_batch_ptrs(self._h, len(ns), _u32(n), m.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                   Op, Ep, int(bool(oriented))))
         self._batch = len(ns)
 
     def batch_graph(self, g: int, with_triangles: bool = True) -> dict:

@@ -130,6 +130,11 @@ int nuc34_graph_dims(const nuc34_ctx* ctx, uint32_t* n, uint64_t* m_oriented);
  * many-graphs workload. */
 int nuc34_load_batch(nuc34_ctx* ctx, uint32_t count, const uint32_t* n, const uint64_t* m, const uint32_t* O,
                      const uint32_t* E, int oriented);
+/* The same union from one (offsets, edges) pointer pair per graph (E[g] may be
+ * NULL when m[g] = 0): each graph is uploaded straight into its place, with no
+ * host-side concatenation. No counterpart in the reference. */
+int nuc34_load_batch_ptrs(nuc34_ctx* ctx, uint32_t count, const uint32_t* n, const uint64_t* m,
+                          const uint32_t* const* O, const uint32_t* const* E, int oriented);
 /* Graph g of the loaded batch: its first vertex and first oriented edge in the
  * union and, once triangles are computed, its triangle slice [tri_first,
  * tri_first + tri_count) of the union's (a,b)-ordered list. The union's edge

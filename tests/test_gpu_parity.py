@@ -271,6 +271,35 @@ def test_batch_matches_single_graphs(variant):
             assert np.array_equal(k_single, ref["k"]) and np.array_equal(t_single, ref["tris"]), j
 
 
+def test_batch_concatenated_and_pointer_loads_agree():
+    """nuc34_load_batch (graphs back to back) and nuc34_load_batch_ptrs (one
+    pointer pair per graph, an edgeless graph among them) load the same union."""
+    import ctypes as C
+    gs = [(np.asarray(g.O, np.uint32), np.asarray(g.E, np.uint32)) for g in
+          (nuc34.gen_batch_graph(3, i) for i in range(6))]
+    gs.insert(2, (np.zeros(4, np.uint32), np.zeros(0, np.uint32)))  # three isolated vertices
+    res = []
+    for mode in ("concatenated", "pointers"):
+        with nuc34.Context() as ctx:
+            if mode == "pointers":
+                ctx.load_batch(gs)
+            else:
+                n = np.array([len(O) - 1 for O, _ in gs], np.uint32)
+                m = np.array([len(E) for _, E in gs], np.uint64)
+                O = np.concatenate([O for O, _ in gs])
+                E = np.concatenate([E for _, E in gs])
+                rc = nuc34._lib.nuc34_load_batch(ctx._h, len(gs), nuc34._u32(n),
+                                                 m.ctypes.data_as(C.POINTER(C.c_uint64)), nuc34._u32(O),
+                                                 nuc34._u32(E), 0)
+                assert rc == nuc34.OK
+                ctx._batch = len(gs)
+            st = ctx.decompose("auto")
+            res.append((ctx.copy_triangles(), ctx.copy_k(), ctx.batch_slices(), st["four_cliques"]))
+    (t0, k0, s0, q0), (t1, k1, s1, q1) = res
+    assert q0 == q1 and np.array_equal(t0, t1) and np.array_equal(k0, k1)
+    assert all(np.array_equal(a, b) for a, b in zip(s0, s1))
+
+
 @pytest.mark.parametrize("case", ["offsets", "range", "self_loop", "unsorted", "asymmetric"])
 def test_malformed_csr_rejected(case):
     """Malformed undirected input is rejected by the device check (which reads
