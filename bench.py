@@ -350,8 +350,9 @@ def main():
                     help="first make a device-to-device copy of this many MiB (counter calibration under --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (graphs shard by graph or by "
-              "vertex range as DESIGN.md describes, but this build runs on one device)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: batches and served graphs "
+              "would shard by graph and one large graph by vertex range of its lowest vertex, but this build "
+              "runs on one device", file=sys.stderr)
         sys.exit(2)
     import nuc34
 
