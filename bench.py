@@ -351,7 +351,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: batches and served graphs "
-              "would shard by graph and one large graph by vertex range of its lowest vertex, but this build "
+              "would shard by graph and one large graph by vertex range of its lowest vertex; the library "
               "runs on one device", file=sys.stderr)
         sys.exit(2)
     import nuc34
