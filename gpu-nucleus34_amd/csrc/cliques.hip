@@ -366,7 +366,9 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
   __shared__ TileLds lds[kWPB];
   const unsigned w = threadIdx.x / kWave;
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * kWave < s.T; tile += nw) {
+  // tiles in XCD-contiguous runs: consecutive triangles share their lowest
+  // vertex's rows, which then stay in one XCD's L2
+  for (uint64_t tile = static_cast<uint64_t>(xcd_block()) * kWPB + w; tile * kWave < s.T; tile += nw) {
     const uint64_t t = tile * kWave + lane_id();
     const bool valid = t < s.T;
     // the lists stream out past the cache (they are read back a round at a
@@ -781,7 +783,7 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
 void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st) {
   if (!s.T) return;
   const uint64_t tiles = (s.T + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for(tiles, kWPB, grid_cap ? grid_cap : 1u << 15)), dim3(kBlock), 0, st, s,
+  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for_xcd(tiles, kWPB, grid_cap ? grid_cap : 1u << 15)), dim3(kBlock), 0, st, s,
                      koff, lists);
   NUC34_LAUNCH_CHECK();
 }

@@ -47,6 +47,15 @@ inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 1u <
   return static_cast<unsigned>(g);
 }
 
+// grid_for rounded to a multiple of the 8 XCDs (up, or down at the cap), for
+// kernels that walk their tiles in xcd_block() order with a grid-strided loop.
+inline unsigned grid_for_xcd(uint64_t items, unsigned per_block, unsigned cap = 1u << 20) {
+  const unsigned g = grid_for(items, per_block, cap);
+  if (g < 8) return g;
+  const unsigned up = (g + 7u) & ~7u;
+  return up <= cap ? up : g & ~7u;  // never above the cap
+}
+
 // ---- device-side helpers -------------------------------------------------
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
@@ -56,6 +65,17 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup number renumbered so that each of the 8 XCDs gets a contiguous run:
+// the hardware deals workgroups to the XCDs in turn, and each XCD has its own
+// L2, so neighbouring tiles that read the same rows then share one L2. It is a
+// permutation of [0, gridDim.x) (the identity when the grid is not a multiple
+// of 8): results never depend on it.
+__device__ __forceinline__ unsigned xcd_block() {
+  constexpr unsigned kXcds = 8;
+  if (gridDim.x % kXcds) return blockIdx.x;
+  return (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
 }
 
 // Number of set bits of `mask` below this lane (mbcnt on the 64-bit ballot).
