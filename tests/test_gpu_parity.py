@@ -558,3 +558,23 @@ def test_random_small_graphs_vs_oracle(kind):
             assert np.array_equal(k, ref["k"]), (kind, it, variant, parts)
             if ref["Q"]:
                 assert st["k_max"] == ref["K_max"] and st["rounds"] == ref["rounds"], (kind, it, variant)
+
+
+@pytest.mark.parametrize("exe", ["nucleus34", "nucleus34_ondemand"])
+def test_cli_full_size_kfile(tmp_path, exe):
+    """The drop-in CLIs at full size (R-MAT scale 16): the k file they write
+    hashes to the reference host path's, and the JSON record carries its counts."""
+    p = gen("preset", "rmat16", out=tmp_path / "rmat16.edges")
+    exp = HASHES["rmat16"]
+    out = tmp_path / "rmat16.k"
+    args = [os.path.join(BUILD, exe), p] + (["4", str(out)] if exe == "nucleus34" else [str(out)])
+    print(f"[cli] {exe} rmat16", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, check=True, timeout=240)
+    rec = json.loads(r.stdout)["dataset"]
+    assert "error" not in rec, rec
+    assert rec["triangle_count"] == exp["T"] and rec["four_clique_count"] == exp["Q"] and rec["K_max"] == exp["K_max"]
+    h = hashlib.sha256()
+    with open(out, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    assert h.hexdigest() == exp["kfile_sha256"]
