@@ -59,6 +59,10 @@ struct nuc34_ctx {
   DevBuf kv;
   nuc34_stats st{};
   nuc34_tuning tune{};  // 0 = defaults
+  // peel polling: pinned host copies of the two control slots after each
+  // captured launch, and the events that mark them written
+  PeelCtl* ctl_host = nullptr;  // 2 launches x 2 slots
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // per-kernel event timers
   struct KTimer {
     const char* name = nullptr;
@@ -411,13 +415,18 @@ void run_peel(nuc34_ctx* c, int variant) {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
     if (variant != NUC34_DIRECT) launch_live_rows(v, c->WA, c->WC, c->s);
     struct Captured {  // destroyed on every way out, errors included
+      hipStream_t s = nullptr;
       hipGraph_t graph = nullptr;
       hipGraphExec_t exec = nullptr;
       ~Captured() {
-        if (exec) (void)hipGraphExecDestroy(exec);
+        if (exec) {
+          (void)hipStreamSynchronize(s);  // no launch of it may still be queued
+          (void)hipGraphExecDestroy(exec);
+        }
         if (graph) (void)hipGraphDestroy(graph);
       }
     } cap;
+    cap.s = c->s;
     if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {
       try {
         enqueue_rounds(per_launch);
@@ -437,19 +446,41 @@ void run_peel(nuc34_ctx* c, int variant) {
     }
     const hipGraphExec_t exec = cap.exec;
     const uint64_t max_launches = 2 * T / per_launch + 16;  // every second round fixes >= 1 triangle
-    for (uint64_t l = 0;; ++l) {
-      ktime(c, K_PEEL, per_launch, [&] {
-        if (exec) NUC34_HIP(hipGraphLaunch(exec, c->s));
-        else enqueue_rounds(per_launch);
-      });
-      NUC34_HIP(hipMemcpyAsync(hs, ctlb.as<void>(), sizeof(hs), hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
-      // the last round written (the higher sequence number) holds the final
-      // counts; the round that finishes marks both slots done
-      h = hs[0].seq >= hs[1].seq ? hs[0] : hs[1];
-      if (h.done) break;
-      if (l > max_launches) throw std::logic_error("peel did not finish");
+    if (!c->ctl_host) {
+      NUC34_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->ctl_host), 4 * sizeof(PeelCtl), hipHostMallocDefault));
+      for (hipEvent_t& e : c->poll_ev) NUC34_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    // One launch stays queued behind the one being checked, so the device never
+    // waits for the host between launches; rounds after the last are no-ops.
+    // Each launch is followed by a copy of both control slots into its own
+    // pinned pair and an event the host waits on.
+    uint64_t launches = 0;
+    auto launch = [&] {
+      if (exec) NUC34_HIP(hipGraphLaunch(exec, c->s));
+      else enqueue_rounds(per_launch);
+      PeelCtl* dst = c->ctl_host + 2 * (launches & 1);
+      NUC34_HIP(hipMemcpyAsync(dst, ctlb.as<void>(), 2 * sizeof(PeelCtl), hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipEventRecord(c->poll_ev[launches & 1], c->s));
+      ++launches;
+    };
+    ktime(c, K_PEEL, 0, [&] {
+      launch();
+      for (uint64_t l = 0;; ++l) {
+        launch();
+        NUC34_HIP(hipEventSynchronize(c->poll_ev[l & 1]));
+        const PeelCtl* got = c->ctl_host + 2 * (l & 1);
+        hs[0] = got[0];
+        hs[1] = got[1];
+        // the last round written (the higher sequence number) holds the final
+        // counts; the round that finishes marks both slots done
+        h = hs[0].seq >= hs[1].seq ? hs[0] : hs[1];
+        if (h.done) break;
+        if (l > max_launches) throw std::logic_error("peel did not finish");
+      }
+      // the launch still queued must end before its graph is destroyed
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    });
+    c->kt[K_PEEL].launches += launches * static_cast<uint64_t>(per_launch);
   });
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
   c->st.rounds = h.round;
@@ -626,6 +657,9 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
     if (k.a) (void)hipEventDestroy(k.a);
     if (k.b) (void)hipEventDestroy(k.b);
   }
+  for (hipEvent_t& e : c->poll_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->s) (void)hipStreamDestroy(c->s);
