@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nuc34.h"
@@ -63,6 +64,9 @@ struct nuc34_ctx {
   // captured launch, and the events that mark them written
   PeelCtl* ctl_host = nullptr;  // 2 launches x 2 slots
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
+  // results back to the host: two pinned staging halves and their events
+  char* stage = nullptr;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   // per-kernel event timers
   struct KTimer {
     const char* name = nullptr;
@@ -112,6 +116,106 @@ double timed(nuc34_ctx* c, F&& f) {
   float ms = 0;
   NUC34_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   return ms;
+}
+
+constexpr size_t kStageBytes = size_t(32) << 20;  // each of the two pinned halves
+
+// memcpy with up to 8 host threads (large blocks only).
+void par_memcpy(char* dst, const char* src, size_t bytes) {
+  const size_t kMin = size_t(4) << 20;
+  const unsigned nt = bytes < 2 * kMin ? 1u : static_cast<unsigned>(std::min<size_t>(8, bytes / kMin));
+  if (nt == 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; ++t) {
+    const size_t lo = t * part, n = lo < bytes ? std::min(part, bytes - lo) : 0;
+    if (n) th.emplace_back([=] { std::memcpy(dst + lo, src + lo, n); });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto& x : th) x.join();
+}
+
+// Device to host through the two pinned halves: the device fills one while
+// host threads move the other into the caller's buffer (pageable memory would
+// otherwise be copied at a fraction of the link's rate). Work already queued
+// on the context stream runs first.
+void to_host(nuc34_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  if (bytes <= (size_t(1) << 20)) {
+    NUC34_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    return;
+  }
+  if (!c->stage) {
+    NUC34_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->stage), 2 * kStageBytes, hipHostMallocDefault));
+    for (hipEvent_t& e : c->stage_ev) NUC34_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const size_t nchunks = (bytes + kStageBytes - 1) / kStageBytes;
+  auto enqueue = [&](size_t i) {
+    const size_t off = i * kStageBytes, n = std::min(kStageBytes, bytes - off);
+    NUC34_HIP(hipMemcpyAsync(c->stage + (i & 1) * kStageBytes, static_cast<const char*>(src) + off, n,
+                             hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipEventRecord(c->stage_ev[i & 1], c->s));
+  };
+  enqueue(0);
+  for (size_t i = 0; i < nchunks; ++i) {
+    if (i + 1 < nchunks) enqueue(i + 1);  // into the half the host finished with last step
+    NUC34_HIP(hipEventSynchronize(c->stage_ev[i & 1]));
+    const size_t off = i * kStageBytes, n = std::min(kStageBytes, bytes - off);
+    par_memcpy(static_cast<char*>(dst) + off, c->stage + (i & 1) * kStageBytes, n);
+  }
+}
+
+// Host to device for one contiguous device range fed from several host
+// segments (graphs' arrays back to back), through the same two pinned halves:
+// host threads fill one half while the device copies the other. Returns with
+// the copies queued on the context stream; a half is rewritten only after its
+// previous copy has finished.
+void to_device(nuc34_ctx* c, void* dst, const std::vector<std::pair<const void*, size_t>>& segs) {
+  size_t total = 0;
+  for (const auto& sg : segs) total += sg.second;
+  if (!total) return;
+  if (total <= (size_t(1) << 20)) {
+    size_t off = 0;
+    for (const auto& sg : segs) {
+      if (sg.second)
+        NUC34_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, sg.first, sg.second, hipMemcpyHostToDevice, c->s));
+      off += sg.second;
+    }
+    return;
+  }
+  if (!c->stage) {
+    NUC34_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->stage), 2 * kStageBytes, hipHostMallocDefault));
+    for (hipEvent_t& e : c->stage_ev) NUC34_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  unsigned half = 0;
+  size_t fill = 0, off = 0;
+  NUC34_HIP(hipEventSynchronize(c->stage_ev[half]));
+  auto flush = [&] {
+    NUC34_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, c->stage + half * kStageBytes, fill,
+                             hipMemcpyHostToDevice, c->s));
+    NUC34_HIP(hipEventRecord(c->stage_ev[half], c->s));
+    off += fill;
+    fill = 0;
+    half ^= 1u;
+    NUC34_HIP(hipEventSynchronize(c->stage_ev[half]));
+  };
+  for (const auto& sg : segs) {
+    const char* p = static_cast<const char*>(sg.first);
+    size_t left = sg.second;
+    while (left) {
+      const size_t n = std::min(left, kStageBytes - fill);
+      par_memcpy(c->stage + half * kStageBytes + fill, p, n);
+      fill += n;
+      p += n;
+      left -= n;
+      if (fill == kStageBytes) flush();
+    }
+  }
+  if (fill) flush();
 }
 
 enum KernelId { K_TRI_BITS, K_TRI_EMIT, K_SUPPORT, K_K4_LISTS, K_PEEL, K_COUNT };
@@ -659,6 +763,9 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
   }
   for (hipEvent_t& e : c->poll_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t& e : c->stage_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stage) (void)hipHostFree(c->stage);
   if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -806,17 +913,15 @@ int load_batch(nuc34_ctx* c, uint32_t count, const uint32_t* n, const uint64_t* 
       DevBuf Oin = c->alloc.get((nv + count) * 4, "batch offsets");
       DevBuf Ein = c->alloc.get(ne * 4 + 4, "batch edges");
       DevBuf firsts = c->alloc.get((count + 1) * 16, "batch boundaries");
-      if (!src.Op) {
-        NUC34_HIP(hipMemcpyAsync(Oin.as<void>(), src.O, (nv + count) * 4, hipMemcpyHostToDevice, c->s));
-        if (ne) NUC34_HIP(hipMemcpyAsync(Ein.as<void>(), src.E, ne * 4, hipMemcpyHostToDevice, c->s));
-      } else {  // each graph straight into its place: no host-side concatenation
-        for (uint32_t g = 0; g < count; ++g) {
-          NUC34_HIP(hipMemcpyAsync(Oin.as<uint32_t>() + vfirst[g] + g, src.Op[g], (n[g] + 1ull) * 4,
-                                   hipMemcpyHostToDevice, c->s));
-          if (m[g])
-            NUC34_HIP(hipMemcpyAsync(Ein.as<uint32_t>() + efirst[g], src.Ep[g], m[g] * 4, hipMemcpyHostToDevice, c->s));
-        }
+      // the graphs' arrays land back to back in Oin and Ein, each taken from
+      // where it lies (no host-side concatenation)
+      std::vector<std::pair<const void*, size_t>> oseg(count), eseg(count);
+      for (uint32_t g = 0; g < count; ++g) {
+        oseg[g] = {src.og(g, vfirst), (n[g] + 1ull) * 4};
+        eseg[g] = {m[g] ? src.eg(g, efirst) : nullptr, m[g] * 4};
       }
+      to_device(c, Oin.as<void>(), oseg);
+      to_device(c, Ein.as<void>(), eseg);
       NUC34_HIP(hipMemcpyAsync(firsts.as<void>(), vfirst.data(), (count + 1) * 8, hipMemcpyHostToDevice, c->s));
       NUC34_HIP(hipMemcpyAsync(firsts.as<uint64_t>() + count + 1, efirst.data(), (count + 1) * 8,
                                hipMemcpyHostToDevice, c->s));
@@ -955,9 +1060,8 @@ int nuc34_copy_triangles(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t*
       const uint64_t n = std::min(chunk, count - done);
       launch_tri_abc(c->triA.as<uint32_t>() + first + done, c->triBC.as<uint64_t>() + first + done, n,
                      buf.as<uint32_t>(), c->s);
-      NUC34_HIP(hipMemcpyAsync(abc + 3 * done, buf.as<void>(), n * 12, hipMemcpyDeviceToHost, c->s));
+      to_host(c, abc + 3 * done, buf.as<void>(), n * 12);  // the buffer is reused only after this returns
     }
-    NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;
   });
 }
@@ -979,10 +1083,7 @@ int nuc34_support(nuc34_ctx* c, uint64_t* Q) {
 int nuc34_copy_support(nuc34_ctx* c, int32_t* support) {
   if (!c || !c->has_sup) return fail(c, NUC34_E_STATE, "support not computed");
   return guard(c, [&]() -> int {
-    if (c->T) {
-      NUC34_HIP(hipMemcpyAsync(support, c->sup.as<void>(), c->T * 4, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
-    }
+    if (c->T) to_host(c, support, c->sup.as<void>(), c->T * 4);
     return NUC34_OK;
   });
 }
@@ -1042,10 +1143,7 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
 int nuc34_copy_k(nuc34_ctx* c, int32_t* k) {
   if (!c || !c->has_k) return fail(c, NUC34_E_STATE, "decomposition not run");
   return guard(c, [&]() -> int {
-    if (c->T) {
-      NUC34_HIP(hipMemcpyAsync(k, c->kv.as<void>(), c->T * 4, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
-    }
+    if (c->T) to_host(c, k, c->kv.as<void>(), c->T * 4);
     return NUC34_OK;
   });
 }
