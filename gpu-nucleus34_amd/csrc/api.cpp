@@ -838,8 +838,8 @@ int nuc34_load_csr(nuc34_ctx* c, const uint32_t* O, const uint32_t* E, uint32_t 
     c->st = nuc34_stats{};
     DevBuf Ou = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "input offsets");
     DevBuf Eu = c->alloc.get(m * 4 + 4, "input edges");
-    NUC34_HIP(hipMemcpyAsync(Ou.as<void>(), O, (static_cast<uint64_t>(n) + 1) * 4, hipMemcpyHostToDevice, c->s));
-    if (m) NUC34_HIP(hipMemcpyAsync(Eu.as<void>(), E, m * 4, hipMemcpyHostToDevice, c->s));
+    to_device(c, Ou.as<void>(), {{O, (static_cast<uint64_t>(n) + 1) * 4}});
+    if (m) to_device(c, Eu.as<void>(), {{E, m * 4}});
     if (!oriented) {
       DevBuf bad = c->alloc.get(4, "check flag");
       launch_fill_u32(bad.as<uint32_t>(), 1, kNoVertex, c->s);
