@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -653,10 +654,10 @@ int nuc34_graph_load(const char* path, nuc34_graph** out, char* err, size_t errl
 
 int nuc34_graph_from_pairs(const uint32_t* src, const uint32_t* dst, uint64_t count, nuc34_graph** out) {
   try {
-    auto* g = new nuc34_graph;
+    auto g = std::make_unique<nuc34_graph>();
     g->lg.g = csr_from_pairs(src, dst, count);
     g->lg.lines = count;
-    *out = g;
+    *out = g.release();
     return NUC34_OK;
   } catch (const std::exception&) {
     return NUC34_E_INVALID;
@@ -696,10 +697,10 @@ int graph_from_generated(const gen::Edges& e, nuc34_graph** out) {
     s[i] = e[i].first;
     d[i] = e[i].second;
   }
-  auto* g = new nuc34_graph;
+  auto g = std::make_unique<nuc34_graph>();
   g->lg.g = csr_from_pairs(s.data(), d.data(), s.size());
   g->lg.lines = e.size();
-  *out = g;
+  *out = g.release();
   return NUC34_OK;
 }
 }  // namespace

@@ -6,7 +6,7 @@
 #include <fstream>
 #include <iterator>
 #include <limits>
-#include <parallel/algorithm>
+#include <omp.h>
 #include <sstream>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -63,6 +63,117 @@ unsigned long stoul_like(const std::string& tok) {
 
 }  // namespace
 
+namespace {
+
+// Edges of one slice of the input, one (src << 32 | dst) word per kept edge.
+struct EdgeSlice {
+  std::vector<uint64_t> pairs;
+  uint32_t max_node = 0;
+  uint64_t lines = 0;
+};
+
+void scan_edge_lines(const char* b, const char* end, EdgeSlice& out) {
+  out.pairs.reserve(static_cast<size_t>(end - b) / 12);
+  while (b < end) {
+    const void* nl = std::memchr(b, '\n', static_cast<size_t>(end - b));
+    const char* le = nl ? static_cast<const char*>(nl) : end;
+    const char* q = b;
+    b = nl ? le + 1 : end;
+    ++out.lines;
+    while (q < le && is_space(*q)) ++q;
+    uint32_t s = 0, d = 0;
+    bool ovf = false;
+    // A line without a leading number, or without a second token, leaves the
+    // reference's destination uninitialised; such lines are skipped here.
+    if (!parse_u32(q, le, s, ovf) || ovf) continue;
+    while (q < le && is_space(*q)) ++q;
+    if (q >= le) continue;
+    if (!parse_u32(q, le, d, ovf)) d = 0;  // failed conversion stores 0
+    if (s == d) continue;
+    out.max_node = std::max(out.max_node, std::max(s, d));
+    out.pairs.push_back((static_cast<uint64_t>(s) << 32) | d);
+  }
+}
+
+// Undirected CSR over max_node + 1 vertices from edge slices. Both
+// directions of every edge become (src << 32 | dst) keys, partitioned by
+// src >> shift into at most 4096 buckets of consecutive sources (per-slice
+// histograms, so the scatter needs no atomics); each bucket is small enough
+// to sort and de-duplicate in cache, and the buckets' vertex ranges are
+// disjoint, so offsets and targets are written bucket by bucket. The result
+// equals sorting and de-duplicating all keys at once.
+Csr csr_from_slices(std::vector<EdgeSlice>& slices, uint32_t max_node) {
+  if (max_node == std::numeric_limits<uint32_t>::max())
+    throw std::runtime_error("Vertices id are larger than defined type.");
+  const int64_t ns = static_cast<int64_t>(slices.size());
+  const uint64_t n = static_cast<uint64_t>(max_node) + 1;
+  int shift = 0;
+  while ((n - 1) >> shift >= 4096) ++shift;
+  const int64_t nb = static_cast<int64_t>(((n - 1) >> shift) + 1);
+  // at[j * (nb + 1) + b]: first key slot of slice j in bucket b.
+  std::vector<uint64_t> at(static_cast<size_t>(ns) * (nb + 1), 0);
+#pragma omp parallel for schedule(static, 1)
+  for (int64_t j = 0; j < ns; ++j) {
+    uint64_t* h = at.data() + j * (nb + 1);
+    for (uint64_t e : slices[j].pairs) {
+      ++h[(e >> 32) >> shift];
+      ++h[(e & 0xffffffffu) >> shift];
+    }
+  }
+  std::vector<uint64_t> bucket(static_cast<size_t>(nb) + 1, 0);
+  for (int64_t b = 0; b < nb; ++b) {
+    uint64_t pos = bucket[b];
+    for (int64_t j = 0; j < ns; ++j) {
+      const uint64_t c = at[j * (nb + 1) + b];
+      at[j * (nb + 1) + b] = pos;
+      pos += c;
+    }
+    bucket[b + 1] = pos;
+  }
+  std::vector<uint64_t> keys(bucket[nb]);
+#pragma omp parallel for schedule(static, 1)
+  for (int64_t j = 0; j < ns; ++j) {
+    uint64_t* cur = at.data() + j * (nb + 1);
+    for (uint64_t e : slices[j].pairs) {
+      const uint64_t s = e >> 32, d = e & 0xffffffffu;
+      keys[cur[s >> shift]++] = e;
+      keys[cur[d >> shift]++] = (d << 32) | s;
+    }
+    std::vector<uint64_t>().swap(slices[j].pairs);
+  }
+  std::vector<uint64_t> kept(static_cast<size_t>(nb) + 1, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t b = 0; b < nb; ++b) {
+    uint64_t* lo = keys.data() + bucket[b];
+    uint64_t* hi = keys.data() + bucket[b + 1];
+    std::sort(lo, hi);
+    kept[b + 1] = static_cast<uint64_t>(std::unique(lo, hi) - lo);
+  }
+  for (int64_t b = 0; b < nb; ++b) kept[b + 1] += kept[b];
+  if (kept[nb] > std::numeric_limits<uint32_t>::max())
+    throw std::runtime_error("Edges size are larger than defined type.");
+  Csr g;
+  g.n = static_cast<uint32_t>(n);
+  g.O.resize(n + 1);
+  g.E.resize(kept[nb]);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t b = 0; b < nb; ++b) {
+    const uint64_t* k = keys.data() + bucket[b];
+    const uint64_t cnt = kept[b + 1] - kept[b];
+    const uint64_t v0 = static_cast<uint64_t>(b) << shift;
+    const uint64_t v1 = std::min(n, v0 + (uint64_t(1) << shift));
+    uint64_t i = 0;
+    for (uint64_t v = v0; v < v1; ++v) {
+      g.O[v] = static_cast<uint32_t>(kept[b] + i);
+      while (i < cnt && (k[i] >> 32) == v) g.E[kept[b] + i] = static_cast<uint32_t>(k[i]), ++i;
+    }
+  }
+  g.O[n] = static_cast<uint32_t>(kept[nb]);
+  return g;
+}
+
+}  // namespace
+
 LoadedGraph parse_edge_text(const char* data, size_t len, bool mtx_header) {
   const char* p = data;
   const char* end = data + len;
@@ -102,44 +213,31 @@ LoadedGraph parse_edge_text(const char* data, size_t len, bool mtx_header) {
     if (nV > std::numeric_limits<uint32_t>::max())
       throw std::runtime_error("Vertices id are larger than defined type.");
   }
-  std::vector<uint64_t> keys;
-  keys.reserve(len / 6);
-  uint32_t max_node = 0;
-  uint64_t lines = 0;
-  while (next_line(lb, le)) {
-    ++lines;
-    const char* q = lb;
-    while (q < le && is_space(*q)) ++q;
-    uint32_t s = 0, d = 0;
-    bool ovf = false;
-    // A line without a leading number, or without a second token, leaves the
-    // reference's destination uninitialised; such lines are skipped here.
-    if (!parse_u32(q, le, s, ovf) || ovf) continue;
-    while (q < le && is_space(*q)) ++q;
-    if (q >= le) continue;
-    if (!parse_u32(q, le, d, ovf)) d = 0;  // failed conversion stores 0
-    if (s == d) continue;
-    max_node = std::max(max_node, std::max(s, d));
-    keys.push_back((static_cast<uint64_t>(s) << 32) | d);
-    keys.push_back((static_cast<uint64_t>(d) << 32) | s);
+  // The body is cut into slices that each start right after a newline, one
+  // per thread (slices of at least 1 MiB), and scanned independently; a line
+  // is wholly inside one slice, so the counts add up to a serial scan's.
+  const size_t body = static_cast<size_t>(end - p);
+  const size_t min_slice = size_t(1) << 20;
+  const int threads = std::max(1, std::min<int>(omp_get_max_threads(),
+                                                static_cast<int>(body / min_slice)));
+  std::vector<const char*> cut(static_cast<size_t>(threads) + 1, end);
+  cut[0] = p;
+  for (int j = 1; j < threads; ++j) {
+    const char* at = std::max(cut[j - 1], p + body / threads * j);
+    const void* nl = at < end ? std::memchr(at, '\n', static_cast<size_t>(end - at)) : nullptr;
+    cut[j] = nl ? static_cast<const char*>(nl) + 1 : end;
   }
-  __gnu_parallel::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  if (keys.size() > std::numeric_limits<uint32_t>::max())
-    throw std::runtime_error("Edges size are larger than defined type.");
+  std::vector<EdgeSlice> slices(static_cast<size_t>(threads));
+#pragma omp parallel for schedule(static, 1) num_threads(threads)
+  for (int j = 0; j < threads; ++j) scan_edge_lines(cut[j], cut[j + 1], slices[j]);
+
   LoadedGraph out;
-  out.lines = lines;
-  Csr& g = out.g;
-  g.n = max_node + 1;
-  if (max_node == std::numeric_limits<uint32_t>::max())
-    throw std::runtime_error("Vertices id are larger than defined type.");
-  g.O.assign(static_cast<size_t>(g.n) + 1, 0);
-  g.E.resize(keys.size());
-  for (size_t i = 0; i < keys.size(); ++i) {
-    g.O[(keys[i] >> 32) + 1]++;
-    g.E[i] = static_cast<uint32_t>(keys[i]);
+  uint32_t max_node = 0;
+  for (const EdgeSlice& s : slices) {
+    out.lines += s.lines;
+    max_node = std::max(max_node, s.max_node);
   }
-  for (size_t v = 0; v < g.n; ++v) g.O[v + 1] += g.O[v];
+  out.g = csr_from_slices(slices, max_node);
   return out;
 }
 
@@ -211,27 +309,23 @@ LoadedGraph load_dataset(const std::string& path) {
 }
 
 Csr csr_from_pairs(const uint32_t* s, const uint32_t* d, size_t count) {
-  std::vector<uint64_t> keys;
-  keys.reserve(2 * count);
+  const size_t per = size_t(1) << 20;
+  const size_t parts = std::max<size_t>(1, std::min<size_t>(omp_get_max_threads(), count / per));
+  std::vector<EdgeSlice> slices(parts);
+#pragma omp parallel for schedule(static, 1)
+  for (int64_t j = 0; j < static_cast<int64_t>(parts); ++j) {
+    EdgeSlice& sl = slices[j];
+    const size_t lo = count * j / parts, hi = count * (j + 1) / parts;
+    sl.pairs.reserve(hi - lo);
+    for (size_t i = lo; i < hi; ++i) {
+      if (s[i] == d[i]) continue;
+      sl.max_node = std::max(sl.max_node, std::max(s[i], d[i]));
+      sl.pairs.push_back((static_cast<uint64_t>(s[i]) << 32) | d[i]);
+    }
+  }
   uint32_t max_node = 0;
-  for (size_t i = 0; i < count; ++i) {
-    if (s[i] == d[i]) continue;
-    max_node = std::max(max_node, std::max(s[i], d[i]));
-    keys.push_back((static_cast<uint64_t>(s[i]) << 32) | d[i]);
-    keys.push_back((static_cast<uint64_t>(d[i]) << 32) | s[i]);
-  }
-  __gnu_parallel::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  Csr g;
-  g.n = max_node + 1;
-  g.O.assign(static_cast<size_t>(g.n) + 1, 0);
-  g.E.resize(keys.size());
-  for (size_t i = 0; i < keys.size(); ++i) {
-    g.O[(keys[i] >> 32) + 1]++;
-    g.E[i] = static_cast<uint32_t>(keys[i]);
-  }
-  for (size_t v = 0; v < g.n; ++v) g.O[v + 1] += g.O[v];
-  return g;
+  for (const EdgeSlice& sl : slices) max_node = std::max(max_node, sl.max_node);
+  return csr_from_slices(slices, max_node);
 }
 
 Csr orient_host(const Csr& und) {

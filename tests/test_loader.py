@@ -112,3 +112,54 @@ def test_pairs_match_text(tmp_path):
     a = nuc34.graph_from_pairs(s, d)
     b = nuc34.load_dataset(str(p))
     assert np.array_equal(a.O, b.O) and np.array_equal(a.E, b.E)
+
+
+@pytest.mark.parametrize("ext", [".edges", ".mtx"])
+def test_large_text_slices_match_oracle(tmp_path, ext):
+    """Texts of several MiB are scanned in newline-aligned slices by parallel
+    threads; ids span more than 4096 vertices so the CSR build uses several
+    source buckets. Result must equal the serial oracle parse."""
+    rng = random.Random(11)
+    out = ["% c", "9000 9000 0"] if ext == ".mtx" else ["# c", "1 2"]
+    forms = 0
+    for i in range(400_000):
+        s, d = rng.randrange(0, 9000), rng.randrange(0, 9000)
+        if i % 97 == 0:
+            s = d = rng.randrange(0, 9000)       # self loop
+        if i % 1013 == 0:
+            out.append("# not a comment here")    # no leading number: skipped
+            forms += 1
+        if i % 5 == 0:
+            out.append(f"{s}\t{d} 0.5\r")
+        elif i % 7 == 0:
+            out.append(f"  {d}  {s}")             # reversed duplicate direction
+        else:
+            out.append(f"{s} {d}")
+    p = tmp_path / f"big{ext}"
+    p.write_text("\n".join(out), newline="")   # no trailing newline
+    assert p.stat().st_size > 4 << 20 and forms > 0
+    g = nuc34.load_dataset(str(p))
+    O, E = oracle34.OracleGraph.load(str(p), orient=False).csr()
+    assert np.array_equal(g.O, O) and np.array_equal(g.E, E)
+
+
+def test_large_pairs_match_numpy():
+    """More than a million pairs are split across threads; the CSR equals a
+    numpy sort/unique of both directions with self loops dropped."""
+    rng = np.random.default_rng(5)
+    cnt = 3_000_000
+    s = (rng.zipf(1.6, cnt) % 200_000).astype(np.uint32)
+    d = rng.integers(0, 200_000, cnt, dtype=np.uint32)
+    g = nuc34.graph_from_pairs(s, d)
+    keep = s != d
+    a = np.concatenate([s[keep], d[keep]]).astype(np.uint64)
+    b = np.concatenate([d[keep], s[keep]]).astype(np.uint64)
+    keys = np.unique((a << np.uint64(32)) | b)
+    n = int(max(s[keep].max(), d[keep].max())) + 1
+    src = (keys >> np.uint64(32)).astype(np.int64)
+    O = np.zeros(n + 1, np.uint64)
+    np.add.at(O, src + 1, 1)
+    O = np.cumsum(O).astype(np.uint32)
+    assert g.n == n
+    assert np.array_equal(g.O, O)
+    assert np.array_equal(g.E, (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32))
