@@ -530,26 +530,32 @@ __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[
 // already at most level).
 __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3], const PeelBufs& p,
                                          int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+  bool own = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) own = own && sq[k] != kGone && !(sq[k] == kCur && q[k] < t);
+  // the three decrements are all issued before any of their results is used
+  bool live[3];
+  int32_t old[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    if (sq[k] == kGone) return;
-    if (sq[k] == kCur && q[k] < t) return;
+    live[k] = own && sq[k] == kAlive;
+    if (live[k]) old[k] = atomic_sub_i32(&p.dd[q[k]], 1);
   }
+  uint32_t pushes = 0;  // bit k: q[k] fell from level+1 to level
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    bool push = false;
-    if (sq[k] == kAlive) {
-      push = atomic_sub_i32(&p.dd[q[k]], 1) == level + 1;
-      if (push) {
-        atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
-        p.kv[q[k]] = level;
-      }
+    if (live[k] && old[k] == level + 1) {
+      pushes |= 1u << k;
+      atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
+      p.kv[q[k]] = level;
     }
-    // one append per wave: the frontier counter is a single hot address
-    bool got;
-    const uint32_t slot = wave_append(ncount, push, &got);
-    if (got) nxt[slot] = q[k];
   }
+  // one append per wave for every push of every lane: the frontier counter is
+  // a single hot address
+  uint32_t slot = wave_append3(ncount, static_cast<uint32_t>(__popc(pushes)));
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if ((pushes >> k) & 1u) nxt[slot++] = q[k];
 }
 
 __device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,

@@ -119,6 +119,20 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, bo
   return base + lanes_below(mask);
 }
 
+// Wave-aggregated append of n in [0, 3] slots per active lane: returns the
+// lane's first slot of *counter; one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append3(uint32_t* counter, uint32_t n) {
+  const uint64_t b0 = __ballot(n & 1u);
+  const uint64_t b1 = __ballot((n >> 1) & 1u);
+  const uint32_t total = static_cast<uint32_t>(__popcll(b0) + 2 * __popcll(b1));
+  if (!total) return 0;
+  const unsigned leader = static_cast<unsigned>(__builtin_ctzll(b0 | b1));
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomic_add_u32(counter, total);
+  base = __shfl(base, static_cast<int>(leader));
+  return base + lanes_below(b0) + 2 * lanes_below(b1);
+}
+
 // Wave-wide inclusive prefix sum of a 32-bit value.
 __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
   const unsigned lane = lane_id();
