@@ -710,11 +710,21 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
     const uint32_t t = cur[i];
     const uint64_t b = koff[t], e = koff[t + 1];
-    for (uint64_t j = b + lane; j < e; j += kWave) {
-      const uint32_t* q = lists + 3 * j;
-      const uint32_t qq[3] = {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
-                              __builtin_nontemporal_load(q + 2)};
-      destroy_k4(t, qq, p, level, nxt, ncount);
+    // two entries per lane and pass: both entries' loads, then both entries'
+    // state reads, are issued before either is waited on
+    for (uint64_t j = b + lane; j < e; j += 2 * kWave) {
+      const bool two = j + kWave < e;
+      uint32_t qa[3], qb[3] = {0u, 0u, 0u}, sa[3], sb[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) qa[k] = __builtin_nontemporal_load(lists + 3 * j + k);
+      if (two) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) qb[k] = __builtin_nontemporal_load(lists + 3 * (j + kWave) + k);
+      }
+      read_states(p, qa, sa);
+      if (two) read_states(p, qb, sb);
+      debit_k4(t, qa, sa, p, level, nxt, ncount);
+      if (two) debit_k4(t, qb, sb, p, level, nxt, ncount);
     }
   }
   prepare_out_slot(out, done);
