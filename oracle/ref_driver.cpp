@@ -1,7 +1,7 @@
 // Driver for the reference's own host path (graph.hpp + cliques.hpp + bucket_ex.hpp),
 // compiled from the headers where they lie under the reference tree by oracle/Makefile
 // (target `ref`). The reference ships these templates but no main that composes
-// them; this file composes them as the survey's host pipeline does:
+// them; this file composes them in the order the reference's CUDA mains run the same steps:
 //   graph_t::from_edges/deserialize -> make_oriented -> make_split_parts
 //   -> cliques_tri_par -> graph_t<..,2>(tris) -> cliques_qud_par
 //   -> graph_t<..,3>(quds, T, undirected) -> bucket_ex::peeling_par -> support()
