@@ -243,6 +243,7 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
   if (c.variant != NUC34_DIRECT && c.variant != NUC34_ONDEMAND && c.variant != NUC34_AUTO_FIT) c.variant = NUC34_AUTO;
   if (c.large_fraction <= 0 || c.large_fraction >= 1) c.large_fraction = 0.75;
   if (!c.large_edges) c.large_edges = 4ull << 20;
+  if (!c.large_wedges) c.large_wedges = 1ull << 28;
   if (!c.max_batch_edges) c.max_batch_edges = 256ull << 20;
   if (!c.max_batch_graphs) c.max_batch_graphs = 4096;
   if (!c.large_workers) c.large_workers = 2;
@@ -297,6 +298,15 @@ int nuc34_serve_submit(nuc34_server* s, const uint32_t* O, const uint32_t* E, ui
   j->O.assign(O, O + n + 1);
   if (m) j->E.assign(E, E + m);
   j->info.n = n;
+  // wedges through each vertex, d(d-1)/2 (one term fits 64 bits: d < 2^32),
+  // summed until the threshold is met; w stays below the threshold meanwhile
+  bool large = m >= s->cfg.large_edges;
+  for (uint64_t v = 0, w = 0; v < n && !large; ++v) {
+    const uint64_t d = O[v + 1] - O[v];
+    const uint64_t t = d ? d * (d - 1) / 2 : 0;
+    if (t >= s->cfg.large_wedges - w) large = true;
+    else w += t;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return NUC34_E_STATE;
   j->id = s->next_id++;
@@ -304,7 +314,7 @@ int nuc34_serve_submit(nuc34_server* s, const uint32_t* O, const uint32_t* E, ui
   *job = j->id;
   Job* raw = j.get();
   s->jobs.emplace(raw->id, std::move(j));
-  s->queue[m >= s->cfg.large_edges ? 1 : 0].push_back(raw);
+  s->queue[large ? 1 : 0].push_back(raw);
   s->st.submitted++;
   s->cv_work.notify_all();
   return NUC34_OK;

@@ -32,7 +32,8 @@ class Nuc34Error(RuntimeError):
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("large_fraction", C.c_double),
                 ("large_edges", C.c_uint64), ("max_batch_edges", C.c_uint64), ("max_batch_graphs", C.c_uint32),
-                ("variant", C.c_int), ("batch_window_ms", C.c_double), ("large_workers", C.c_uint32)]
+                ("variant", C.c_int), ("batch_window_ms", C.c_double), ("large_workers", C.c_uint32),
+                ("large_wedges", C.c_uint64)]
 
 
 class JobInfo(C.Structure):
@@ -427,10 +428,10 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, large_fraction: float = 0.0, large_edges: int = 0,
                  max_batch_edges: int = 0, max_batch_graphs: int = 0, variant="auto", batch_window_ms: float = 0.0,
-                 large_workers: int = 0):
+                 large_workers: int = 0, large_wedges: int = 0):
         cfg = ServeConfig(device, hbm_budget, large_fraction, large_edges, max_batch_edges, max_batch_graphs,
                           VARIANTS[variant] if isinstance(variant, str) else int(variant), batch_window_ms,
-                          large_workers)
+                          large_workers, large_wedges)
         self._h = C.c_void_p()
         rc = _lib.nuc34_serve_create(C.byref(cfg), C.byref(self._h))
         if rc != OK:

@@ -221,7 +221,8 @@ int nuc34_device_info(int device, char* buf, size_t len);
  * arrival order: one small-job lane and `large_workers` large-job workers. The small-job lane packs every queued job that fits its
  * caps into one disjoint-union batch (one pass of every kernel, shared peel
  * rounds) as soon as it is free; the large-job lane runs graphs at or above
- * `large_edges` one at a time, so they never hold small jobs back. Admission
+ * `large_edges` edge entries or `large_wedges` wedges one at a time, so they
+ * never hold small jobs back. Admission
  * is by HBM footprint: each lane holds its share of the budget, a batch runs
  * in the configured variant (NUC34_AUTO: on demand, O(T) memory), and a batch
  * that runs out of its lane's budget is split in halves and retried.
@@ -239,6 +240,9 @@ typedef struct nuc34_serve_config {
   double batch_window_ms;    /* small-job lane: wait this long after its oldest queued job
                                 arrived before starting a batch (0 = start at once) */
   uint32_t large_workers;    /* workers (contexts, streams) for large jobs (0 = 2, at most 64) */
+  uint64_t large_wedges;     /* wedges, sum over vertices of d(d-1)/2, at which a job is large
+                                whatever its edge count (0 = 2^28): a graph with hubs costs more
+                                than its edge count says */
 } nuc34_serve_config;
 
 typedef struct nuc34_job_info {

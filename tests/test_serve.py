@@ -122,3 +122,34 @@ def test_serve_auto_fit_variant_per_batch():
         infos = check_jobs(srv, jobs, gs)
     assert {i["variant"] for i in infos} <= {nuc34.DIRECT, nuc34.ONDEMAND}
     assert any(i["variant"] == nuc34.DIRECT for i in infos)
+
+
+def test_serve_small_jobs_run_beside_a_large_one():
+    """Continuous batching: small jobs that arrive while a large job is on the
+    device are admitted by their own lane and run beside it (their start and
+    end times fall inside the large job's), and every result, the large one's
+    included, equals the graph's decomposition alone."""
+    print("[serve] R-MAT s16 large job with small jobs arriving beside it", flush=True)
+    import time
+    big = nuc34.gen_preset("rmat16")
+    pool = [nuc34.gen_batch_graph(4, i) for i in range(0, 24, 3)]
+    with nuc34.Server(hbm_budget=32 << 30) as srv:
+        jb = srv.submit(big)
+        jobs, gs = [], []
+        while len(jobs) < 40:
+            g = pool[len(jobs) % len(pool)]
+            jobs.append(srv.submit(g))
+            gs.append(g)
+            if srv.wait(jb, 0) is not None:
+                break
+            time.sleep(0.002)
+        ib = srv.wait(jb, 120000)
+        infos = check_jobs(srv, jobs, gs)
+        assert ib is not None and ib["status"] == nuc34.OK, ib
+        tb, kb = srv.take(jb, ib["triangles"])
+    inside = [i for i in infos if ib["t_start"] <= i["t_start"] and i["t_done"] <= ib["t_done"]]
+    print(f"[serve] {len(inside)} of {len(infos)} small jobs ran inside the large job's "
+          f"{1e3 * (ib['t_done'] - ib['t_start']):.0f} ms", flush=True)
+    assert inside
+    t1, k1, _ = single(big)
+    assert np.array_equal(tb, t1) and np.array_equal(kb, k1)
