@@ -59,6 +59,10 @@ WORKLOADS = {
                         "desc": "R-MAT scale 16, edge factor 16, seed 1, on-demand variant"},
     "er1000": {"gen": ["preset", "er1000"], "variant": "direct", "parts": 1,
                "desc": "Erdos-Renyi G(1000, 0.1), seed 0, direct variant"},
+    "rmat18": {"gen": ["preset", "rmat18"], "variant": "direct", "parts": 1,
+               "desc": "R-MAT scale 18, edge factor 16, seed 1, direct variant (the serving mix's large job)"},
+    "rmat18-ondemand": {"gen": ["preset", "rmat18"], "variant": "ondemand", "parts": 1,
+                        "desc": "R-MAT scale 18, edge factor 16, seed 1, on-demand variant"},
     "rmat20k30": {"gen": ["preset", "rmat20k30"], "variant": "ondemand", "parts": 1,
                   "desc": "R-MAT scale 20, edge factor 16, seed 2, + 64 planted K30, on-demand variant"},
 }
