@@ -628,6 +628,10 @@ std::string validate_csr(const uint32_t* O, const uint32_t* E, uint32_t n, bool 
   return std::string();
 }
 
+// The message of this thread's last failed nuc34_ctx_create (there is no
+// context to hold it).
+thread_local std::string tls_create_error;
+
 }  // namespace
 
 extern "C" {
@@ -732,6 +736,7 @@ int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
   }
 }
 
+
 int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
   if (!out) return NUC34_E_INVALID;
   auto* c = new nuc34_ctx;
@@ -744,7 +749,8 @@ int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out) {
     return NUC34_OK;
   });
   if (rc != NUC34_OK) {
-    nuc34_ctx_destroy(c);  // releases whatever was created
+    tls_create_error = c->err;  // nuc34_last_error(NULL) reports it
+    nuc34_ctx_destroy(c);       // releases whatever was created
     *out = nullptr;
     return rc;
   }
@@ -774,7 +780,10 @@ void nuc34_ctx_destroy(nuc34_ctx* c) {
   delete c;
 }
 
-const char* nuc34_last_error(const nuc34_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* nuc34_last_error(const nuc34_ctx* c) {
+  if (c) return c->err.c_str();
+  return tls_create_error.empty() ? "null context" : tls_create_error.c_str();
+}
 
 }  // extern "C"
 

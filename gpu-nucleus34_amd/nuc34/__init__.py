@@ -255,7 +255,7 @@ class Context:
         self._h = C.c_void_p()
         rc = _lib.nuc34_ctx_create(device, hbm_budget, C.byref(self._h))
         if rc != OK:
-            raise Nuc34Error(rc, "nuc34_ctx_create failed")
+            raise Nuc34Error(rc, "nuc34_ctx_create failed: " + (_lib.nuc34_last_error(None) or b"").decode())
 
     def close(self):
         if self._h:

@@ -113,6 +113,8 @@ int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
 /* hbm_budget: bytes the context may hold on the device (0 = no limit). */
 int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out);
 void nuc34_ctx_destroy(nuc34_ctx* ctx);
+/* The message of the context's last failure; with ctx NULL, that of this
+ * thread's last failed nuc34_ctx_create (for example no device). */
 const char* nuc34_last_error(const nuc34_ctx* ctx);
 
 /* Uploads a CSR. oriented == 0: undirected (both directions, lists sorted by

@@ -67,3 +67,20 @@ def test_struct_layouts_match_header(tmp_path):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f in fs:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_failed_context_creation_names_its_cause():
+    """Without a usable device nuc34_ctx_create fails with a status, and
+    nuc34_last_error(NULL) gives that failure's message (no context holds it)."""
+    import ctypes
+    import nuc34
+    lib = nuc34.lib()
+    ctx = ctypes.c_void_p()
+    rc = lib.nuc34_ctx_create(0, ctypes.c_uint64(0), ctypes.byref(ctx))
+    if rc == 0:  # a device is present: nothing to check
+        lib.nuc34_ctx_destroy(ctx)
+        return
+    assert not ctx.value
+    msg = lib.nuc34_last_error(None)
+    msg = msg.decode() if isinstance(msg, bytes) else ctypes.cast(msg, ctypes.c_char_p).value.decode()
+    assert msg and msg != "null context", msg
