@@ -291,6 +291,8 @@ def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: in
                 if info["status"] != nuc34.OK or not (np.array_equal(k, ek) and np.array_equal(tris, et)):
                     bad.append((key, info["status"], info["error"]))
                 results.append((key[0], info["t_submit"], info["t_done"], info["batch_graphs"]))
+                if len(results) % 1000 == 0:
+                    print(f"[serve] {len(results)} of {len(arrivals)} jobs collected", file=sys.stderr, flush=True)
         th = threading.Thread(target=collect)
         th.start()
         t0 = srv.now()
