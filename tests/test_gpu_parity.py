@@ -578,3 +578,45 @@ def test_cli_full_size_kfile(tmp_path, exe):
         for chunk in iter(lambda: f.read(1 << 24), b""):
             h.update(chunk)
     assert h.hexdigest() == exp["kfile_sha256"]
+
+
+def _same_as_oracle(g, ref, tag):
+    for variant, parts in (("direct", 1), ("ondemand", 1), ("direct", 3)):
+        tris, sup, k, st = gpu_run(g, variant, parts)
+        assert st["triangles"] == ref["T"] and st["four_cliques"] == ref["Q"], (tag, variant, parts)
+        assert np.array_equal(tris, ref["tris"]) and np.array_equal(sup, ref["support"]), (tag, variant, parts)
+        assert np.array_equal(k, ref["k"]), (tag, variant, parts)
+        if ref["Q"]:
+            assert st["k_max"] == ref["K_max"] and st["rounds"] == ref["rounds"], (tag, variant, parts)
+
+
+def test_random_graph_sweep():
+    """1000 small random graphs (the four shapes above, one seed each): every
+    run equals the oracle in both variants and with parts > 1."""
+    for i in range(1000):
+        kind = ("er", "cliques", "hub", "spread")[i % 4]
+        g = random_graph(np.random.default_rng(100000 + i), kind)
+        ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 4)
+        _same_as_oracle(g, ref, (kind, 100000 + i))
+        if (i + 1) % 200 == 0:
+            print(f"[sweep] {i + 1} random graphs equal the oracle", flush=True)
+
+
+def test_rmat_and_batch_mix_sweep():
+    """200 R-MAT graphs with drawn scale (8-12), edge factor (4-32), skew and
+    planted cliques, and 100 graphs of the batch mix from drawn seeds: every
+    run equals the oracle."""
+    rng = np.random.default_rng(7)
+    for i in range(300):
+        if i < 200:
+            sc, ef, a = int(rng.integers(8, 13)), int(rng.integers(4, 33)), float(rng.uniform(0.45, 0.7))
+            b = (1 - a) * float(rng.uniform(0.3, 0.45))
+            args = (sc, ef, a, b, b, 1000 + i, int(rng.integers(0, 4)), int(rng.integers(5, 16)))
+            g = nuc34.gen_rmat(*args)
+        else:
+            args = (int(rng.integers(10, 1000)), int(rng.integers(0, 256)))
+            g = nuc34.gen_batch_graph(*args)
+        ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 8)
+        _same_as_oracle(g, ref, args)
+        if (i + 1) % 50 == 0:
+            print(f"[sweep] {i + 1} R-MAT / mix graphs equal the oracle (last T={ref['T']} Q={ref['Q']})", flush=True)
