@@ -87,15 +87,18 @@ def test_device_orientation_matches_reference_order(tmp_path):
     assert np.array_equal(O, Oo) and np.array_equal(E, Eo)
 
 
-@pytest.mark.parametrize("n", [4, 6, 11, 40])
+@pytest.mark.parametrize("n", [4, 6, 11, 40, 400])
 def test_clique_closed_form(n):
+    """K_n: C(n,3) triangles, C(n,4) K4s, k = n-3 everywhere, one round. K_400
+    has 1.05 G K4s (more than 2^30), every one with all four members dying in
+    the same round."""
     pairs = np.array(list(itertools.combinations(range(n), 2)), dtype=np.uint32)
     g = nuc34.graph_from_pairs(pairs[:, 0], pairs[:, 1])
     for variant in ("direct", "ondemand"):
         tris, sup, k, st = gpu_run(g, variant)
         assert st["triangles"] == n * (n - 1) * (n - 2) // 6
         assert st["four_cliques"] == n * (n - 1) * (n - 2) * (n - 3) // 24
-        assert (k == n - 3).all() and st["k_max"] == n - 3
+        assert (k == n - 3).all() and st["k_max"] == n - 3 and st["rounds"] == 1
 
 
 def test_no_triangles_and_empty():
