@@ -170,6 +170,23 @@ def test_full_size_rmat20k30(tmp_path):
         assert hashlib.sha256(tris.tobytes()).hexdigest() == exp["tri_raw_sha256"]
 
 
+def test_full_size_rmat18(tmp_path):
+    """R-MAT scale 18 (the serving mix's large job: 83 M triangles, 2.3 G
+    four-cliques), both variants: the raw k and triangle arrays hash to the
+    oracle's, and T, Q, K_max and the peel rounds agree."""
+    exp = HASHES["rmat18"]
+    g = nuc34.load_dataset(gen("preset", "rmat18", out=tmp_path / "rmat18.edges"))
+    for variant in ("ondemand", "direct"):
+        print(f"[full] rmat18 {variant}", flush=True)
+        with nuc34.Context() as ctx:
+            ctx.load_graph(g)
+            st = ctx.decompose(variant)
+            assert (st["triangles"], st["four_cliques"], st["k_max"]) == (exp["T"], exp["Q"], exp["K_max"])
+            assert st["rounds"] == exp["rounds"] and st["unpeeled"] == 0
+            assert hashlib.sha256(ctx.copy_k().tobytes()).hexdigest() == exp["k_raw_sha256"], variant
+            assert hashlib.sha256(ctx.copy_triangles().tobytes()).hexdigest() == exp["tri_raw_sha256"], variant
+
+
 def test_cli_matches_golden(tmp_path):
     for exe, name in (("nucleus34", "rmat8.edges"), ("nucleus34_ondemand", "rmat7k8.edges"),
                       ("nucleus34", "er40.grh"), ("nucleus34", "star.edges"), ("nucleus34_ondemand", "wheel.edges")):
