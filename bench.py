@@ -1,39 +1,47 @@
 #!/usr/bin/env python3
 """Headline benchmark: (3,4)-nucleus decomposition on one MI355X.
 
-Workload (default): R-MAT scale 16, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05),
-seed 1, symmetrised and deduplicated (15.5 M triangles, 287 M four-cliques),
-direct variant with the per-triangle K4 lists materialised in HBM.
+Workload (default): R-MAT scale 20, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05),
+seed 2, plus 64 planted 30-cliques, symmetrised and deduplicated (424 M triangles,
+17.0 G four-cliques), on-demand variant: the largest single-GPU configuration
+(the direct variant's K4 lists would need 816 GB).
 
 A step = one decomposition from the oriented CSR resident in HBM to k for every
 triangle resident in HBM: triangle enumeration + bit-packed incidence, K4
-support, K4 lists, peel. Between steps the context drops the derived stages
-(buffers return to its reuse cache, so the timed steps allocate nothing).
-value = four-cliques / second = Q / (mean step time).
+support, peel. Between steps the context drops the derived stages (buffers
+return to its reuse cache, so the timed steps allocate nothing).
+value = four-cliques / second = Q / (mean step time), printed only when the raw
+k and triangle arrays of the last step hash to the golden values (the
+reference's on-demand algorithm on the same graph) and T, Q, K_max and the
+round count agree; otherwise value is null and the exit status is 1.
 
 Prints one JSON line with the driver's contract, plus:
-  roofline      the dominant kernel's algorithmic bytes per launch over its
-                average launch time (HIP events on the context stream);
-                traffic = its HBM bytes per launch from two rocprofv3 --pmc
+  roofline      the dominant kernel group (the peel rounds): algorithmic bytes
+                per launched round (the per-unit formulas of the hot-path survey,
+                with L(t) summed from the per-edge triangle counts) over its
+                average round time from HIP events on the context stream;
+                traffic = its HBM bytes per round from two rocprofv3 --pmc
                 child runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
                 size in the same process;
-  batched       512 graphs of the seeded mixed batch decomposed as one disjoint
-                union versus one decomposition per graph (graphs/s, bit-exact);
   roofline.atomics  the peel's support decrements (device atomics) per second
                 against the measured random-atomic rate of the device;
+  cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
+                the reference headers) on R-MAT scale 16 (the same generator at
+                a size the host path holds: about 64 bytes per four-clique), timed
+                on this host's cores beside the GPU's time on the same file; and
+                the C++ restatement of the reference's on-demand algorithm timed
+                once on the full default workload (profiles/, read if present);
+  batched       512 graphs of the seeded mixed batch decomposed as one disjoint
+                union versus one decomposition per graph (graphs/s, bit-exact);
   device_rates  measured copy GB/s and random 4-byte atomic / load rates;
   config.kernel_gbs      algorithmic GB/s of every timed kernel group;
-  other_variant the other variant (direct <-> on demand) on the same graph, with
-                default launch shapes, and whether its k equals the headline's;
   config.end_to_end_ms   parse, load (host to device, checks, orientation),
-                decomposition and the copy of k and the triangles back;
-  cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
-                the reference headers) timed on this host's cores on a bounded
-                sample (R-MAT scale 14, same generator parameters).
+                decomposition and the copy of k and the triangles back.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import subprocess
@@ -46,11 +54,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "gpu-nucleus34_amd")
 GEN = os.path.join(PKG, "_build", "nuc34_gen")
+HASHES = os.path.join(ROOT, "tests", "golden", "hashes.json")
+C3_CPU = os.path.join(ROOT, "profiles", "rmat20k30_cpu_ondemand.json")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 WORKLOADS = {
+    "rmat20k30": {"gen": ["preset", "rmat20k30"], "variant": "ondemand", "parts": 1,
+                  "desc": "R-MAT scale 20, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 2, + 64 planted K30, "
+                          "on-demand variant"},
     "rmat16": {"gen": ["preset", "rmat16"], "variant": "direct", "parts": 1,
                "desc": "R-MAT scale 16, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 1, direct variant"},
     "rmat16-parts4": {"gen": ["preset", "rmat16"], "variant": "direct", "parts": 4,
@@ -63,9 +76,22 @@ WORKLOADS = {
                "desc": "R-MAT scale 18, edge factor 16, seed 1, direct variant (the serving mix's large job)"},
     "rmat18-ondemand": {"gen": ["preset", "rmat18"], "variant": "ondemand", "parts": 1,
                         "desc": "R-MAT scale 18, edge factor 16, seed 1, on-demand variant"},
-    "rmat20k30": {"gen": ["preset", "rmat20k30"], "variant": "ondemand", "parts": 1,
-                  "desc": "R-MAT scale 20, edge factor 16, seed 2, + 64 planted K30, on-demand variant"},
 }
+
+# Launch shapes chosen by nuc34_autotune on one MI355X (bench.py --autotune
+# prints the tuner's choice); shapes change speed, never results.
+TUNED = {
+    "rmat20k30": {"peel_debit_grid": 8192, "peel_tile_cap": 8, "peel_rounds_per_launch": 64, "tri_tile_edges": 16,
+                  "peel_finish_grid": 1024},
+    "rmat16": {"peel_debit_grid": 8192, "peel_tile_cap": 64, "peel_rounds_per_launch": 32, "k4_list_grid_cap": 8192,
+               "tri_tile_edges": 16, "peel_finish_grid": 512},
+    "batch": {"peel_debit_grid": 2048, "peel_tile_cap": 32, "peel_rounds_per_launch": 16, "k4_list_grid_cap": 32768,
+              "tri_tile_edges": 64, "peel_finish_grid": 512},
+}
+
+
+def tuned_for(workload: str) -> dict | None:
+    return TUNED.get(workload.split("-")[0] if workload.startswith("rmat16") else workload)
 
 
 def graph_stats(O: np.ndarray, E: np.ndarray) -> dict:
@@ -75,24 +101,26 @@ def graph_stats(O: np.ndarray, E: np.ndarray) -> dict:
 
 
 def algorithmic_bytes(kernel: str, g: dict, st: dict) -> float:
-    """Bytes an ideal implementation of `kernel` must move per launch."""
-    T, Q, m, n = st["triangles"], st["four_cliques"], g["m"], g["n"]
-    if kernel == "tri_enum_bits":
-        # per edge: src, head, O[u], O[u+1], O[v], O[v+1]; each candidate w of
-        # N+(v); the owner lists; one 4-byte read-modify-write per role bit
-        return 24 * m + 4 * g["P"] + 4 * m + 3 * 8 * T
+    """Bytes per launch of `kernel` by the per-unit formulas of the hot-path
+    survey: n, m, T, Q; D = support decrements applied; L(t) = l(a)+l(b)+l(c)
+    with l(e) the triangles on edge e, so sum_t L(t) = sum_e l(e)^2 (g["sum_l2"]);
+    the merge intersection reads 4 bytes per entry of N+(u) and N+(v) for every
+    oriented edge (u,v): sum_u d+(u)^2 + P."""
+    T, Q, m, n, D = st["triangles"], st["four_cliques"], g["m"], g["n"], st["debits"]
+    lists = 4 * (g["sum_dplus2"] + g["P"])
+    enum_bytes = 12 * T + 8 * g["sum_l2"]  # sum_t (12 + 8 L(t))
+    if kernel == "tri_enum_bits":  # pass 1 of 2: list reads, CSR; the triangle records come in pass 2
+        return lists + 8 * (n + 1) + 4 * m
     if kernel == "tri_enum_emit":
-        return 24 * m + 4 * g["P"] + 4 * m + T * (12 + 8)
+        return lists + 12 * T + 8 * (n + 1) + 4 * m
     if kernel == "support_bits":
-        return T * (12 + 12) + 2 * 4 * m  # triangle record in, three counts out, rows read once
-    if kernel == "k4_lists":
-        # triangle record and list offset in, four 12-byte entries per K4 out
-        return T * (12 + 8) + Q * 4 * 12
+        return enum_bytes + 4 * T
+    if kernel == "k4_lists":  # direct: the K4 records and the triangle -> K4 incidence written
+        return enum_bytes + 4 * T + 16 * Q + 32 * Q
     if kernel == "peel_rounds":
-        # every K4 entry once (read from the lists, or produced by the on-demand
-        # enumeration), one 8-byte atomic read-modify-write per debit (at most
-        # 3 per K4), support/state/k/frontier traffic per triangle
-        return 4 * Q * 12 + 3 * Q * 8 + T * 20
+        if st["variant"] == 0:  # direct
+            return 8 * (T + 1) + 16 * Q + 32 * Q + 16 * Q + 8 * D + 4 * T
+        return enum_bytes + 8 * D + 4 * T  # on demand: every triangle re-enumerates its K4s once, at death
     return 0.0
 
 
@@ -131,23 +159,25 @@ def calibration_copy(mib: int) -> None:
             raise RuntimeError(f"HIP error {rc} in the calibration copy")
 
 
-def run_traffic(workload: str, group: str, launches: float, tmp: str, tuning: dict | None,
+def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str, tuning: dict | None,
                 calib_mib: int = 1024) -> dict | None:
     """HBM bytes per launch of `group` from two rocprofv3 --pmc runs of a
-    one-step bench (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
-    by what the same counter reads for a device-to-device copy of `calib_mib` MiB
-    made first in the same process. The child runs with the launch shapes of the
-    timed run (`tuning`), not a tuning pass of its own. None when rocprofv3 is
+    one-step bench on the same edge file (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the same counter reads for a device-to-device
+    copy of `calib_mib` MiB made first in the same process. The child runs with
+    the launch shapes of the timed run (`tuning`). None when rocprofv3 is
     unavailable or fails."""
     prof = "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-batch",
-             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--no-other-variant", "--workload", workload,
-             *(["--tuning", json.dumps(tuning)] if tuning else ["--no-autotune"]), "--copy-calibration-mib", str(calib_mib)]
+             "--no-cpu-baseline", "--no-traffic", "--no-rates", "--no-other-variant", "--no-verify",
+             "--workload", workload, "--edges", edges,
+             *(["--tuning", json.dumps(tuning)] if tuning else []), "--copy-calibration-mib", str(calib_mib)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(tmp, counter)
+        print(f"[bench] rocprofv3 --pmc {counter} child run", file=sys.stderr, flush=True)
         try:
             # the profiler can exit nonzero after writing complete results, so
             # the database decides, not the exit status
@@ -190,39 +220,95 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def run_cpu_baseline(tmp: str, threads: int) -> dict:
+def sha256_of(a: np.ndarray) -> str:
+    h = hashlib.sha256()
+    mv = memoryview(np.ascontiguousarray(a)).cast("B")
+    for i in range(0, len(mv), 1 << 28):
+        h.update(mv[i:i + (1 << 28)])
+    return h.hexdigest()
+
+
+def verify(preset: str, st: dict, k: np.ndarray, tris: np.ndarray) -> dict:
+    """The last step's results against the golden values for this graph: T, Q,
+    K_max, rounds and the sha256 of the raw k and (a,b,c) arrays."""
+    gold = json.load(open(HASHES)).get(preset)
+    if not gold or "k_raw_sha256" not in gold:
+        return {"checked": False, "bit_exact": None}
+    got = {"T": st["triangles"], "Q": st["four_cliques"], "K_max": st["k_max"], "rounds": st["rounds"],
+           "k_raw_sha256": sha256_of(k), "tri_raw_sha256": sha256_of(tris)}
+    bad = [f for f in got if f in gold and got[f] != gold[f]]
+    return {"checked": True, "bit_exact": not bad, "mismatched": bad, "k_raw_sha256": got["k_raw_sha256"],
+            "tri_raw_sha256": got["tri_raw_sha256"],
+            "against": "tests/golden/hashes.json (the oracle's on-demand and host restatements; for er1000 and "
+                       "rmat16 also the reference's own host path)"}
+
+
+def run_same_input_pair(nuc34, tmp: str, threads: int) -> dict:
+    """R-MAT scale 16 (seed 1): the reference's host path and the GPU on the same
+    .edges file in the same run, each checked against the golden hashes."""
+    path = os.path.join(tmp, "rmat16.edges")
+    subprocess.run([GEN, "preset", "rmat16", path], check=True, stdout=subprocess.DEVNULL)
+    g = nuc34.load_dataset(path)
+    gpu = {}
+    with nuc34.Context(0) as ctx:
+        ctx.load_graph(g)
+        ctx.set_tuning(**TUNED["rmat16"])
+        for variant in ("direct", "ondemand"):
+            ctx.reset()
+            ctx.decompose(variant)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                ctx.reset()
+                st = ctx.decompose(variant)
+            ms = (time.perf_counter() - t0) / 3 * 1e3
+            ver = verify("rmat16", st, ctx.copy_k(), ctx.copy_triangles())
+            gpu[variant] = {"ms": ms, "value": st["four_cliques"] / (ms / 1e3), "bit_exact": ver["bit_exact"]}
     ref = os.path.join(ROOT, "oracle", "_ref", "ref34")
     kind = "reference"
-    if not os.path.exists(ref):
+    kfile = os.path.join(tmp, "rmat16.ref.k")
+    if os.path.exists(ref):
+        args = [ref, path, kfile, str(threads)]
+    else:  # the reference tree was not there to build it: the oracle's restatement of the same path
         ref = os.path.join(ROOT, "oracle", "_build", "oracle34")
         kind = "port"
-        if not os.path.exists(ref):
-            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    path = os.path.join(tmp, "sample.edges")
-    subprocess.run([GEN, "rmat", "14", "16", "0.57", "0.19", "0.19", "1", path], check=True,
-                   stdout=subprocess.DEVNULL)
-    args = [ref, path, "", str(threads)] if kind == "reference" else [ref, path, "--parts", str(threads)]
+        args = [ref, path, "--parts", str(threads), "--kfile", kfile]
+    print(f"[bench] CPU host path ({kind}) on R-MAT scale 16, {threads} threads", file=sys.stderr, flush=True)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     out = subprocess.run(args, check=True, capture_output=True, text=True, env=env).stdout
     rec = json.loads(out.strip().splitlines()[-1])
     secs = sum(rec.get(k, 0.0) for k in ("tri_s", "tri_graph_s", "qud_s", "qud_graph_s", "peel_s"))
-    return {"value": rec["Q"] / secs, "unit": "4-cliques/s", "cores": threads, "kind": kind, "cpu": cpu_model(),
+    h = hashlib.sha256()
+    with open(kfile, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    gold = json.load(open(HASHES))["rmat16"]
+    cpu_value = rec["Q"] / secs
+    return {"value": cpu_value, "unit": "4-cliques/s", "cores": threads, "kind": kind, "cpu": cpu_model(),
+            "threads_note": "OpenMP loops on `cores` threads; std::execution::par runs serial (libstdc++ without TBB)",
             "phases_s": {k: rec.get(k, 0.0) for k in ("tri_s", "tri_graph_s", "qud_s", "qud_graph_s", "peel_s")},
-            "sample": f"R-MAT scale 14, edge factor 16, same (a,b,c,d), seed 1: T={rec['T']}, Q={rec['Q']}, "
-                      f"K_max={rec['K_max']}; decomposition {secs:.2f} s (load excluded); "
-                      f"{'reference host path compiled from cliques.hpp+bucket_ex.hpp (serial pstl: no TBB)' if kind == 'reference' else 'oracle restatement'}"}
+            "kfile_matches_golden": h.hexdigest() == gold["kfile_sha256"],
+            "sample": f"R-MAT scale 16, edge factor 16, same (a,b,c,d), seed 1 (T={rec['T']}, Q={rec['Q']}, "
+                      f"K_max={rec['K_max']}): the default workload's generator at a size the reference's host path "
+                      f"holds in memory; decomposition {secs:.2f} s, load excluded",
+            "same_input": {"file": "R-MAT scale 16 seed 1 (.edges)", "cpu_s": secs,
+                           "gpu": gpu, "speedup_direct": gpu["direct"]["value"] / cpu_value,
+                           "speedup_ondemand": gpu["ondemand"]["value"] / cpu_value}}
 
 
-def run_batched(nuc34, count: int, seed: int = 3) -> dict:
+def run_batched(nuc34, count: int, seed: int = 3, autotune: bool = False) -> dict:
     """Many-graphs workload: `count` graphs of the seeded mixed batch, decomposed
     as one disjoint union (one pass of every kernel, shared peel rounds) versus one
     decomposition per graph; both include upload, device orientation and the
-    copy of k back to the host. Every
-    graph's slice of the batched result must equal its own decomposition."""
+    copy of k back to the host. Every graph's slice of the batched result must
+    equal its own decomposition."""
     gs = [nuc34.gen_batch_graph(seed, i) for i in range(count)]
     with nuc34.Context(0) as ctx:
-        ctx.load_batch(gs)  # launch shapes tuned on the union, used by both modes
-        tuning, _ = ctx.autotune("auto")
+        ctx.load_batch(gs)  # launch shapes for the union, used by both modes
+        if autotune:
+            tuning, _ = ctx.autotune("auto")
+        else:
+            ctx.set_tuning(**TUNED["batch"])
+            tuning = ctx.tuning()
         ctx.load_batch(gs[:8])  # warm-up of both paths (allocator cache, code objects)
         ctx.decompose("auto")
         ctx.load_graph(gs[0])
@@ -249,7 +335,8 @@ def run_batched(nuc34, count: int, seed: int = 3) -> dict:
             "tuning": tuning}
 
 
-def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: int = 4, pool: int = 256) -> dict:
+def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: int = 4, pool: int = 256,
+                small_workers: int = 0) -> dict:
     """Serving workload: Poisson arrivals at `rate` jobs/s for `seconds`, seed
     `seed`; each job is a graph of the seeded mixed batch (pool of `pool`
     graphs) or, with probability `large_share`, an R-MAT scale 18 edge factor 16
@@ -278,7 +365,7 @@ def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: in
             expect[(b, i)] = (ctx.copy_triangles(), ctx.copy_k())
     results, bad = [], []
     q = queue.Queue()
-    with nuc34.Server(0) as srv:
+    with nuc34.Server(0, small_workers=small_workers) as srv:
         def collect():
             while True:
                 item = q.get()
@@ -330,28 +417,40 @@ def peel_atomics(st: dict, rates: dict | None) -> dict | None:
             "floor_ms": st["debits"] / (peak * 1e9) * 1e3 if peak else None}
 
 
+def c3_cpu_record() -> dict | None:
+    """The C++ restatement of the reference's on-demand algorithm timed once on
+    the default workload on the GPU box's host cores (profiles/), if present."""
+    try:
+        return json.load(open(C3_CPU))
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat16", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rmat20k30", choices=sorted(WORKLOADS))
+    ap.add_argument("--edges", default=None, help="use this .edges file instead of generating the workload's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain comparison")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc runs for roofline.traffic")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--autotune", action="store_true", help="tune launch shapes on the graph instead of the committed ones")
     ap.add_argument("--tuning", default=None,
-                    help="launch shapes as a JSON object of nuc34_tuning fields, used instead of autotuning")
+                    help="launch shapes as a JSON object of nuc34_tuning fields (default: the committed shapes)")
     ap.add_argument("--no-other-variant", action="store_true",
                     help="skip timing the other variant (direct <-> on demand) on the same graph")
     ap.add_argument("--no-rates", action="store_true",
                     help="skip measuring the device's copy bandwidth and random atomic / load rates")
+    ap.add_argument("--no-verify", action="store_true", help="skip the golden-hash check (profiler child runs)")
     ap.add_argument("--serve-seconds", type=float, default=0.0,
                     help="also run the serving workload (Poisson arrivals) for this many seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-large-share", type=float, default=0.05)
+    ap.add_argument("--serve-small-workers", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first make a device-to-device copy of this many MiB (counter calibration under --pmc)")
     args = ap.parse_args()
@@ -365,9 +464,11 @@ def main():
     if args.copy_calibration_mib:
         calibration_copy(args.copy_calibration_mib)
     wl = WORKLOADS[args.workload]
+    preset = wl["gen"][-1]
     with tempfile.TemporaryDirectory() as tmp:
-        path = os.path.join(tmp, "graph.edges")
-        subprocess.run([GEN, *wl["gen"], path], check=True, stdout=subprocess.DEVNULL)
+        path = args.edges or os.path.join(tmp, "graph.edges")
+        if not args.edges:
+            subprocess.run([GEN, *wl["gen"], path], check=True, stdout=subprocess.DEVNULL)
         t_parse = time.perf_counter()
         g = nuc34.load_dataset(path)
         t_parse = time.perf_counter() - t_parse
@@ -375,13 +476,18 @@ def main():
         t_load = time.perf_counter()
         ctx.load_graph(g)  # host to device, checks, orientation on the device
         t_load = time.perf_counter() - t_load
+        del g
         O, E = ctx.oriented_csr()
         gs = graph_stats(O, E)
-        if args.tuning:
-            ctx.set_tuning(**json.loads(args.tuning))
-            tuning = ctx.tuning()
+        del O, E
+        if args.autotune:
+            tuning = ctx.autotune(wl["variant"])[0]
         else:
-            tuning = None if args.no_autotune else ctx.autotune(wl["variant"])[0]
+            shapes = json.loads(args.tuning) if args.tuning else tuned_for(args.workload)
+            if shapes:
+                ctx.set_tuning(**shapes)
+            tuning = ctx.tuning()
+        print(f"[bench] {args.workload}: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
         for _ in range(args.warmup):
             ctx.reset()
             ctx.decompose(wl["variant"], wl["parts"])
@@ -391,17 +497,22 @@ def main():
             ctx.reset()
             steps.append(ctx.decompose(wl["variant"], wl["parts"]))
         t1 = time.perf_counter()
+        st = steps[-1]
         t_copy = time.perf_counter()
         k_main = ctx.copy_k()
-        ctx.copy_triangles()
+        tris = ctx.copy_triangles()
         t_copy = time.perf_counter() - t_copy
+        parity = {"checked": False, "bit_exact": None} if args.no_verify else verify(preset, st, k_main, tris)
+        del tris
+        lcount = ctx.copy_edge_triangles().astype(np.uint64)
+        gs["sum_l2"] = int((lcount * lcount).sum())
+        gs["sum_l"] = int(lcount.sum())
+        del lcount
         other = None
         if not args.no_other_variant:
-            # the other variant on the same graph (default launch shapes), timed the
+            # the other variant on the same graph (same launch shapes), timed the
             # same way; its k must equal the headline variant's
             ov = "ondemand" if wl["variant"] == "direct" else "direct"
-            if tuning:
-                ctx.set_tuning(**{f: 0 for f in tuning})
             try:
                 ctx.reset()
                 ctx.decompose(ov, wl["parts"])
@@ -415,13 +526,19 @@ def main():
                          "k_equal": bool(np.array_equal(ctx.copy_k(), k_main))}
             except nuc34.Nuc34Error as e:  # e.g. the direct lists do not fit
                 other = {"variant": ov, "error": str(e)}
+        del k_main
         ctx.close()
         rates = None if args.no_rates else nuc34.device_rates(0)
-        batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs)
-        serving = (run_serving(nuc34, args.serve_seconds, args.serve_rate, args.serve_large_share)
-                   if args.serve_seconds > 0 else None)
-        cpu = None if args.no_cpu_baseline else run_cpu_baseline(tmp, args.cpu_threads)
-        st = steps[-1]
+        batched = None if args.no_batch else run_batched(nuc34, args.batch_graphs, autotune=args.autotune)
+        serving = (run_serving(nuc34, args.serve_seconds, args.serve_rate, args.serve_large_share,
+                               small_workers=args.serve_small_workers) if args.serve_seconds > 0 else None)
+        cpu = None
+        if not args.no_cpu_baseline:
+            cpu = run_same_input_pair(nuc34, tmp, args.cpu_threads)
+            c3 = c3_cpu_record()
+            if c3 and args.workload == "rmat20k30":
+                c3["speedup_vs_this_run"] = c3["decomp_s"] / ((t1 - t0) / args.steps)
+                cpu["full_workload_restatement"] = c3
         kern = {}
         for s_ in steps:
             for name, k in s_["kernels"].items():
@@ -430,10 +547,11 @@ def main():
                 a["launches"] += k["launches"]
         dom = max(kern, key=lambda k: kern[k]["ms"])
         launches_per_step = kern[dom]["launches"] / args.steps
-        traffic = None if args.no_traffic else run_traffic(args.workload, dom, launches_per_step, tmp, tuning)
+        traffic = None if args.no_traffic else run_traffic(args.workload, path, dom, launches_per_step, tmp, tuning)
 
     ms = (t1 - t0) / args.steps * 1e3
-    value = st["four_cliques"] / (ms / 1e3)
+    exact = parity["bit_exact"]
+    value = st["four_cliques"] / (ms / 1e3) if exact is not False else None
     end_to_end = {"parse_ms": t_parse * 1e3, "load_ms": t_load * 1e3, "decompose_ms": ms,
                   "copy_k_and_triangles_ms": t_copy * 1e3}
     end_to_end["total_ms"] = sum(end_to_end.values())
@@ -443,9 +561,6 @@ def main():
     per_launch_ms = kern[dom]["ms"] / kern[dom]["launches"]
     bytes_per_launch = algorithmic_bytes(dom, gs, st) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
-    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "hashes.json"))).get(wl["gen"][-1], {})
-    parity = all(st[k] == golden[g_] for k, g_ in (("triangles", "T"), ("four_cliques", "Q"), ("k_max", "K_max"))
-                 if g_ in golden) if golden else None
     rec = {
         "metric": "4-cliques/s (decomposition wall time, bit-exact k)",
         "value": value,
@@ -462,16 +577,18 @@ def main():
         "config": {"workload": wl["desc"], "variant": wl["variant"], "parts": wl["parts"],
                    "n": gs["n"], "m_oriented": gs["m"], "triangles": st["triangles"],
                    "four_cliques": st["four_cliques"], "K_max": st["k_max"], "rounds": st["rounds"],
-                   "counts_match_reference": parity, "tuning": tuning,
+                   "sum_l2": gs["sum_l2"], "parity": parity, "tuning": tuning,
                    "phase_ms": {k: round(st[k], 3) for k in ("ms_triangles", "ms_support", "ms_k4_list",
                                                            "ms_incidence", "ms_peel", "ms_total")},
                    "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()},
+                   "kernel_launches_per_step": {k: v["launches"] / args.steps for k, v in kern.items()},
                    "kernel_gbs": kernel_gbs, "end_to_end_ms": {k: round(v, 3) for k, v in end_to_end.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_detail": traffic,
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
+                     "launches_per_step": launches_per_step,
                      "atomics": peel_atomics(st, rates)},
         "device_rates": rates,
         "other_variant": other,
@@ -480,6 +597,9 @@ def main():
         "serving": serving,
     }
     print(json.dumps(rec))
+    if exact is False:
+        print(f"results differ from the golden values: {parity['mismatched']}", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

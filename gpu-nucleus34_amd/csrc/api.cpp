@@ -348,7 +348,8 @@ BitRows bitrows(nuc34_ctx* c) {
 K4Src k4src(nuc34_ctx* c) {
   return K4Src{c->O.as<uint32_t>(),    c->E.as<uint32_t>(),     c->src.as<uint32_t>(), c->inpos.as<uint32_t>(),
                bitrows(c),             c->triA.as<uint32_t>(),  c->triBC.as<uint64_t>(), c->cIdx.as<uint32_t>(),
-               c->ainpos.as<uint32_t>(), c->T,                  nullptr,                 nullptr};
+               c->ainpos.as<uint32_t>(), c->T,                  nullptr,                 nullptr,
+               nullptr};
 }
 
 void run_triangles(nuc34_ctx* c, uint32_t parts) {
@@ -495,12 +496,14 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
               b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
   K4Src v = k4src(c);
-  DevBuf liveA, liveC;
+  DevBuf liveA, liveC, icut;
   if (variant != NUC34_DIRECT) {
-    liveA = c->alloc.get(c->WA * 4 + 4, "live a/b rows");
-    liveC = c->alloc.get(c->WC * 4 + 4, "live c rows");
-    v.LA = liveA.as<uint32_t>();
-    v.LC = liveC.as<uint32_t>();
+    liveA = c->alloc.get(c->WA * 8 + 8, "live a/b rows");
+    liveC = c->alloc.get(c->WC * 8 + 8, "live c rows");
+    icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
+    v.LA = liveA.as<LiveWord>();
+    v.LC = liveC.as<LiveWord>();
+    v.icut = icut.as<uint32_t>();
   }
   // round parity must equal slot parity: groups of even size
   const int per_launch = c->tune.peel_rounds_per_launch ? static_cast<int>(c->tune.peel_rounds_per_launch & ~1u)
@@ -518,7 +521,8 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl h{};
   c->st.ms_peel = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
-    if (variant != NUC34_DIRECT) launch_live_rows(v, c->WA, c->WC, c->s);
+    if (variant != NUC34_DIRECT)
+      launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(), c->s);
     struct Captured {  // destroyed on every way out, errors included
       hipStream_t s = nullptr;
       hipGraph_t graph = nullptr;
@@ -994,7 +998,7 @@ int nuc34_load_batch_ptrs(nuc34_ctx* c, uint32_t count, const uint32_t* n, const
 int nuc34_batch_graph(nuc34_ctx* c, uint32_t g, uint64_t* vertex_first, uint64_t* edge_first, uint64_t* tri_first,
                       uint64_t* tri_count) {
   if (!c || c->batch_v.empty()) return fail(c, NUC34_E_STATE, "no batch loaded");
-  if (g + 1 >= c->batch_v.size()) return fail(c, NUC34_E_INVALID, "graph index out of range");
+  if (static_cast<uint64_t>(g) + 1 >= c->batch_v.size()) return fail(c, NUC34_E_INVALID, "graph index out of range");
   if (vertex_first) *vertex_first = c->batch_v[g];
   if (edge_first) *edge_first = c->batch_e[g];
   if (!tri_first && !tri_count) return NUC34_OK;
@@ -1072,6 +1076,18 @@ int nuc34_copy_triangles(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t*
                      buf.as<uint32_t>(), c->s);
       to_host(c, abc + 3 * done, buf.as<void>(), n * 12);  // the buffer is reused only after this returns
     }
+    return NUC34_OK;
+  });
+}
+
+int nuc34_copy_edge_triangles(nuc34_ctx* c, uint32_t* counts) {
+  if (!c || !c->has_tris) return fail(c, NUC34_E_STATE, "triangles not computed");
+  if (!counts) return fail(c, NUC34_E_INVALID, "null argument");
+  return guard(c, [&]() -> int {
+    if (!c->m) return NUC34_OK;
+    DevBuf cnt = c->alloc.get(c->m * 4, "edge triangle counts");
+    launch_edge_triangles(bitrows(c), c->m, cnt.as<uint32_t>(), c->s);
+    to_host(c, counts, cnt.as<void>(), c->m * 4);
     return NUC34_OK;
   });
 }
@@ -1170,10 +1186,17 @@ int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_
       exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
                                          c->s);
     }
-    DevBuf k4 = c->alloc.get(c->Q * 8 + 8, "owned K4s");
-    launch_k4_owned(k4src(c), k4off.as<uint64_t>(), k4.as<uint64_t>(), c->s);
+    // only the triangles whose owned K4s reach [first, first + count) are walked
+    DevBuf bounds = c->alloc.get(16, "K4 copy bounds");
+    launch_k4_owned_bounds(k4off.as<uint64_t>(), T, first, count, bounds.as<uint64_t>(), c->s);
+    uint64_t tb[2] = {0, 0};
+    NUC34_HIP(hipMemcpyAsync(tb, bounds.as<void>(), 16, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    if (tb[0] > tb[1] || tb[1] >= T) throw std::logic_error("owned K4 offsets inconsistent");
+    DevBuf k4 = c->alloc.get(count * 8, "owned K4s");
+    launch_k4_owned(k4src(c), k4off.as<uint64_t>(), tb[0], tb[1] + 1, first, count, k4.as<uint64_t>(), c->s);
     DevBuf out = c->alloc.get(count * 16, "K4 copy");
-    launch_k4_quads(k4src(c), k4.as<uint64_t>(), first, count, out.as<uint32_t>(), c->s);
+    launch_k4_quads(k4src(c), k4.as<uint64_t>(), 0, count, out.as<uint32_t>(), c->s);
     NUC34_HIP(hipMemcpyAsync(quds, out.as<void>(), count * 16, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;

@@ -59,6 +59,17 @@ __global__ void k_row_counts(BitRows r, uint64_t m, uint32_t* __restrict__ cntA,
   }
 }
 
+__global__ void k_edge_triangles(BitRows r, uint64_t m, uint32_t* __restrict__ cnt) {
+  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t c = 0;
+    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i)
+      c += static_cast<uint32_t>(__popc(r.Aout[i]) + __popc(r.Ain[i].bits));
+    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) c += static_cast<uint32_t>(__popc(r.Cm[i].bits));
+    cnt[e] = c;
+  }
+}
+
 // Running ranks: every Ain word of e gets offA[e] plus the set bits of e's row
 // before it, every Cm word cOff[e] plus the bits before it. Ranks are triangle
 // ids and c-slots, which stay below 2^32 (checked by the caller).
@@ -125,6 +136,12 @@ void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src
 void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s) {
   if (!m) return;
   hipLaunchKernelGGL(k_row_counts, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cntA, cntC);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStream_t s) {
+  if (!m) return;
+  hipLaunchKernelGGL(k_edge_triangles, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cnt);
   NUC34_LAUNCH_CHECK();
 }
 

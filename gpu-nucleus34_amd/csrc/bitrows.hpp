@@ -64,6 +64,8 @@ void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src
                       uint32_t* wc, hipStream_t s);
 // Role counts from the rows: cntA = |Ain|, cntC = |Cm| per edge.
 void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s);
+// Triangles on each edge, in any role: cnt[e] = |Aout| + |Ain| + |Cm| of e.
+void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStream_t s);
 // Running ranks of the Ain words (from offA) and Cm words (from cOff).
 void launch_row_ranks(const BitRows& r, uint64_t m, const uint64_t* offA, const uint64_t* cOff, hipStream_t s);
 

@@ -50,6 +50,7 @@ struct TriRows {
   uint64_t ca;   // Cm row of a (over N-(u)); b's is (jv - jw) * WC words further
   uint64_t cc;   // Cm row of c (over N-(v))
   uint32_t c, ou, ov, jv, jw;  // c-edge; O[u], O[v]; positions of v and w in N+(u)
+  uint32_t cut;  // on-demand peel: in-neighbours of u with a lower id (icut[u]); 0 elsewhere
   uint32_t W;    // words of the rows of u's out-edges
   uint32_t Wv;   // words of the rows of v's out-edges
   uint32_t WC;   // words of the a / b c-role rows
@@ -61,7 +62,9 @@ __device__ __forceinline__ TriRows tri_rows(const K4Src& s, uint32_t t) {
   const uint64_t bc = s.triBC[t];
   const uint32_t b = static_cast<uint32_t>(bc >> 32);
   r.c = static_cast<uint32_t>(bc);
-  r.ou = s.O[s.src[b]];
+  const uint32_t u = s.src[b];
+  r.ou = s.O[u];
+  r.cut = s.icut ? s.icut[u] : 0u;
   r.ov = s.O[s.E[b]];
   r.jv = b - r.ou;
   r.jw = a - r.ou;
@@ -96,6 +99,7 @@ __device__ __forceinline__ TriRows shfl_rows(const TriRows& r, uint32_t src) {
   o.ov = shfl_u32(r.ov, src);
   o.jv = shfl_u32(r.jv, src);
   o.jw = shfl_u32(r.jw, src);
+  o.cut = shfl_u32(r.cut, src);
   o.W = shfl_u32(r.W, src);
   o.Wv = shfl_u32(r.Wv, src);
   o.WC = shfl_u32(r.WC, src);
@@ -108,9 +112,25 @@ struct K4Word {
   uint32_t bits;
   uint32_t Aa, Ab;  // up: b-role words of a and b
   RankWord Ia, Ib;  // up: a-role words of a and b; down: c-role words of a and b
+  uint32_t Da, Db;  // LIVE: dying-plane words of a and b (0 elsewhere)
 };
 
-// LIVE: the K4 bits come from the live rows (the on-demand peel).
+// Bits j of a row word starting at bit position `base` with base + j < p.
+__device__ __forceinline__ uint32_t bits_below(uint32_t p, uint32_t base) {
+  if (p <= base) return 0u;
+  const uint32_t d = p - base;
+  return d >= 32 ? ~0u : (1u << d) - 1u;
+}
+
+// LIVE: the K4 bits come from the live rows (the on-demand peel), without the
+// K4s that a partner on t's own rows owns: a partner dying in this round with
+// a lower triangle index. Triangles sharing their lowest vertex u are ordered
+// by the positions in N+(u) of their highest, then middle vertex, so for t at
+// (jw, jv) and x at bit position jx of an up word:
+//   T{u,v,x} (row b) precedes t iff  v < x ? jx < jw : jv < jw;
+//   T{u,w,x} (row a) precedes t iff  w < x ? jx < jw : jx < jv;
+// and for a down word (x at position i of N-(u), x below u) both row partners
+// have lowest vertex x, and precede t iff id x < id u, i.e. i < icut[u].
 template <bool LIVE>
 __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint32_t wi) {
   K4Word k;
@@ -121,7 +141,18 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     k.Ab = s.rows.Aout[rb];
     k.Ia = s.rows.Ain[ra];
     k.Ib = s.rows.Ain[rb];
-    k.bits = LIVE ? s.LA[ra] & s.LA[rb] : (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
+    if (LIVE) {
+      const LiveWord la = s.LA[ra], lb = s.LA[rb];
+      const uint32_t bw = bits_below(r.jw, wi * 32), bv = bits_below(r.jv, wi * 32);
+      const uint32_t lower_b = (k.Ab & bw) | (~k.Ab & (r.jv < r.jw ? ~0u : 0u));
+      const uint32_t lower_a = (k.Aa & bw) | (~k.Aa & bv);
+      k.bits = la.live & lb.live & ~((lb.dying & lower_b) | (la.dying & lower_a));
+      k.Da = la.dying;
+      k.Db = lb.dying;
+    } else {
+      k.bits = (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
+      k.Da = k.Db = 0;
+    }
   } else {
     // b's row: (jv - jw) * WC words from a's (either sign; 64-bit wraparound)
     const uint64_t ca = r.ca + (wi - r.W);
@@ -129,7 +160,15 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     k.Aa = k.Ab = 0;
     k.Ia = s.rows.Cm[ca];
     k.Ib = s.rows.Cm[cb];
-    k.bits = LIVE ? s.LC[ca] & s.LC[cb] : k.Ia.bits & k.Ib.bits;
+    if (LIVE) {
+      const LiveWord la = s.LC[ca], lb = s.LC[cb];
+      k.bits = la.live & lb.live & ~((la.dying | lb.dying) & bits_below(r.cut, (wi - r.W) * 32));
+      k.Da = la.dying;
+      k.Db = lb.dying;
+    } else {
+      k.bits = k.Ia.bits & k.Ib.bits;
+      k.Da = k.Db = 0;
+    }
   }
   return k;
 }
@@ -191,6 +230,7 @@ __device__ __forceinline__ uint32_t find_slot(const uint32_t* pre, uint32_t key)
   return lo;
 }
 
+template <bool LIVE>
 __device__ __forceinline__ K4Word shfl_word(const K4Word& k, uint32_t src) {
   K4Word o;
   o.bits = shfl_u32(k.bits, src);
@@ -200,6 +240,8 @@ __device__ __forceinline__ K4Word shfl_word(const K4Word& k, uint32_t src) {
   o.Ia.rank = shfl_u32(k.Ia.rank, src);
   o.Ib.bits = shfl_u32(k.Ib.bits, src);
   o.Ib.rank = shfl_u32(k.Ib.rank, src);
+  o.Da = LIVE ? shfl_u32(k.Da, src) : 0u;
+  o.Db = LIVE ? shfl_u32(k.Db, src) : 0u;
   return o;
 }
 
@@ -218,8 +260,10 @@ struct TileLds {
 // (the on-demand peel). The tile's row words are
 // flattened over the lanes 64 at a time, then the set bits of those 64 words
 // (the K4s) are flattened again, one K4 per lane and step, and
-// emit(t, aux, pos, q) runs once per K4 of each triangle. With POS, pos is the K4's index in the
-// triangle's list (up words first, word and bit order); without, pos is 0.
+// emit(t, aux, pos, q, die) runs once per K4 of each triangle. With POS, pos is the K4's index in the
+// triangle's list (up words first, word and bit order); without, pos is 0. With
+// LIVE, bit k of die says that q[k] (k = 0, 1: the partners on t's own rows)
+// dies in the current round; 0 elsewhere.
 template <bool POS, bool LIVE, typename Emit>
 __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, uint64_t aux, TileLds& L,
                                         Emit&& emit) {
@@ -269,7 +313,7 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
     for (uint32_t eb = 0; eb < nk; eb += kWave) {
       const uint32_t e = eb + lane;
       const uint32_t sl = find_slot(L.epre, e < nk ? e : nk - 1);
-      const K4Word kw = shfl_word(k, sl);
+      const K4Word kw = shfl_word<LIVE>(k, sl);
       const uint32_t ti = shfl_u32(i, sl);
       const uint32_t wsl = shfl_u32(wi, sl);
       const uint32_t esl = shfl_u32(epre, sl);
@@ -282,7 +326,9 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
         const uint32_t j = select_bit(kw.bits, r);
         uint32_t q[3];
         k4_partners(s, rt, wsl, kw, j, q);
-        emit(tt, ta, POS ? psl + r : 0u, q);
+        // q[0] sits on b's row, q[1] on a's, at bit j of this word
+        const uint32_t die = LIVE ? (((kw.Db >> j) & 1u) | (((kw.Da >> j) & 1u) << 1)) : 0u;
+        emit(tt, ta, POS ? psl + r : 0u, q, die);
       }
     }
     wave_sync();
@@ -374,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
     // the lists stream out past the cache (they are read back a round at a
     // time much later); the row words of the lookups keep it
     k4_tile<true, false>(s, static_cast<uint32_t>(valid ? t : 0), valid, valid ? koff[t] : 0, lds[w],
-                         [&](uint32_t, uint64_t first, uint32_t pos, const uint32_t q[3]) {
+                         [&](uint32_t, uint64_t first, uint32_t pos, const uint32_t q[3], uint32_t) {
                            uint32_t* o = lists + 3 * (first + pos);
                            __builtin_nontemporal_store(q[0], o);
                            __builtin_nontemporal_store(q[1], o + 1);
@@ -384,24 +430,48 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
 }
 
 // Owned K4s: fourth vertex above all of t, i.e. b-role bits of both a and b.
-__global__ __launch_bounds__(kBlock) void k_k4_owned(K4Src s, const uint64_t* __restrict__ k4off,
+// Only the records [first, first + count) of the canonical list are written,
+// at k4[pos - first], and only triangles [t_begin, t_end) are walked: the
+// ones whose owned K4s reach that window.
+__global__ __launch_bounds__(kBlock) void k_k4_owned(K4Src s, const uint64_t* __restrict__ k4off, uint64_t t_begin,
+                                                     uint64_t t_end, uint64_t first, uint64_t count,
                                                      uint64_t* __restrict__ k4) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < s.T; t += stride) {
+  const uint64_t last = first + count;
+  for (uint64_t t = t_begin + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < t_end; t += stride) {
     const uint32_t a = s.triA[t];
     const uint32_t b = static_cast<uint32_t>(s.triBC[t] >> 32);
     const uint32_t* Aa = s.rows.Aout + s.rows.rowA[a];
     const uint32_t* Ab = s.rows.Aout + s.rows.rowA[b];
     const uint32_t W = static_cast<uint32_t>(s.rows.rowA[a + 1] - s.rows.rowA[a]);
     uint64_t pos = k4off[t];
-    for (uint32_t wi = 0; wi < W; ++wi) {
+    for (uint32_t wi = 0; wi < W && pos < last; ++wi) {
       uint32_t bits = Aa[wi] & Ab[wi];
-      while (bits) {
+      while (bits && pos < last) {
         const uint32_t jx = wi * 32 + static_cast<uint32_t>(__builtin_ctz(bits));
         bits &= bits - 1;
-        k4[pos++] = (t << 32) | jx;
+        if (pos >= first) k4[pos - first] = (t << 32) | jx;
+        ++pos;
       }
     }
+  }
+}
+
+// out[0], out[1]: the triangles holding owned records `first` and
+// `first + count - 1` (the last t with k4off[t] <= index; count >= 1 and
+// first + count <= k4off[T]).
+__global__ void k_k4_owned_bounds(const uint64_t* __restrict__ k4off, uint64_t T, uint64_t first, uint64_t count,
+                                  uint64_t* __restrict__ out) {
+  if (blockIdx.x || threadIdx.x) return;
+  const uint64_t key[2] = {first, first + count - 1};
+  for (int k = 0; k < 2; ++k) {
+    uint64_t lo = 0, hi = T + 1;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (k4off[mid] <= key[k]) lo = mid + 1;
+      else hi = mid;
+    }
+    out[k] = lo - 1;
   }
 }
 
@@ -558,11 +628,38 @@ __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const 
     if ((pushes >> k) & 1u) nxt[slot++] = q[k];
 }
 
-__device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], const PeelBufs& p, int32_t level,
-                                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
-  uint32_t sq[3];
-  read_states(p, q, sq);
-  debit_k4(t, q, sq, p, level, nxt, ncount);
+// On-demand variant: the K4 came from t's live rows, so q[0] and q[1] are not
+// gone, the dying planes say whether each dies in this round (`die`), and the
+// word-level check has already dropped the K4 if one of them owns it. Only
+// q[2]'s state is read. A row partner that is not dying is debited: it is
+// alive or was pushed earlier in this round, and a pushed triangle's k is
+// fixed, so its extra debit changes nothing (it cannot be pushed again).
+__device__ __forceinline__ void destroy_k4_live(uint32_t t, const uint32_t q[3], uint32_t die, const PeelBufs& p,
+                                                int32_t level, uint32_t* __restrict__ nxt,
+                                                uint32_t* __restrict__ ncount) {
+  const uint32_t s2 = tri_state(p.st, q[2]);
+  if (s2 == kGone || (s2 == kCur && q[2] < t)) return;
+  bool live[3];
+  live[0] = !(die & 1u);
+  live[1] = !(die & 2u);
+  live[2] = s2 == kAlive;
+  int32_t old[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (live[k]) old[k] = atomic_sub_i32(&p.dd[q[k]], 1);
+  uint32_t pushes = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (live[k] && old[k] == level + 1) {
+      pushes |= 1u << k;
+      atomic_or_u32(&p.st[q[k] >> 4], kNext << ((q[k] & 15u) * 2u));
+      p.kv[q[k]] = level;
+    }
+  }
+  uint32_t slot = wave_append3(ncount, static_cast<uint32_t>(__popc(pushes)));
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if ((pushes >> k) & 1u) nxt[slot++] = q[k];
 }
 
 // Dying triangles per wave for a frontier of n: spread over every wave of the
@@ -570,6 +667,49 @@ __device__ __forceinline__ void destroy_k4(uint32_t t, const uint32_t q[3], cons
 __device__ __forceinline__ uint32_t tile_width(uint32_t n, uint32_t waves) {
   const uint32_t g = n / waves;
   return g < 1 ? 1u : (g > static_cast<uint32_t>(kWave) ? static_cast<uint32_t>(kWave) : g);
+}
+
+// The live-row words holding triangle t's three bits, and the bits: t is bit
+// jv of a's row, bit jw of b's row (both over N+(u)) and bit inpos(b) of c's
+// c-role row (over N-(v)).
+struct TriSlots {
+  uint64_t w[3];
+  uint32_t bit[3];
+};
+
+__device__ __forceinline__ TriSlots tri_slots(const K4Src& s, uint32_t t) {
+  const uint32_t a = s.triA[t];
+  const uint64_t bc = s.triBC[t];
+  const uint32_t b = static_cast<uint32_t>(bc >> 32), c = static_cast<uint32_t>(bc);
+  const uint32_t ou = s.O[s.src[b]];
+  const uint32_t jv = b - ou, jw = a - ou, ip = s.inpos[b];
+  TriSlots x;
+  x.w[0] = s.rows.rowA[a] + (jv >> 5);
+  x.bit[0] = jv & 31u;
+  x.w[1] = s.rows.rowA[b] + (jw >> 5);
+  x.bit[1] = jw & 31u;
+  x.w[2] = s.rows.rowC[c] + (ip >> 5);
+  x.bit[2] = ip & 31u;
+  return x;
+}
+
+// t joins the current frontier: its bits in the dying planes.
+__device__ __forceinline__ void mark_dying(const K4Src& s, uint32_t t) {
+  const TriSlots x = tri_slots(s, t);
+  atomic_or_u32(&s.LA[x.w[0]].dying, 1u << x.bit[0]);
+  atomic_or_u32(&s.LA[x.w[1]].dying, 1u << x.bit[1]);
+  atomic_or_u32(&s.LC[x.w[2]].dying, 1u << x.bit[2]);
+}
+
+// t is retired: its bits leave both planes (one 64-bit atomic per word).
+__device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
+  const TriSlots x = tri_slots(s, t);
+  LiveWord* const words[3] = {s.LA + x.w[0], s.LA + x.w[1], s.LC + x.w[2]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint64_t bit = 1ull << x.bit[k];
+    atomic_and_u64(reinterpret_cast<uint64_t*>(words[k]), ~(bit | (bit << 32)));
+  }
 }
 
 template <bool UNLINK>
@@ -587,21 +727,13 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
   for (uint64_t i = gtid; i < F; i += gsize) {
     const uint32_t t = cur[i];
     atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
-    if (UNLINK) {
-      const uint32_t a = s.triA[t];
-      const uint64_t bc = s.triBC[t];
-      const uint32_t b = static_cast<uint32_t>(bc >> 32), c = static_cast<uint32_t>(bc);
-      const uint32_t ou = s.O[s.src[b]];
-      const uint32_t jv = b - ou, jw = a - ou, ip = s.inpos[b];
-      atomic_and_u32(&s.LA[s.rows.rowA[a] + (jv >> 5)], ~(1u << (jv & 31)));  // t in a's row
-      atomic_and_u32(&s.LA[s.rows.rowA[b] + (jw >> 5)], ~(1u << (jw & 31)));  // t in b's row
-      atomic_and_u32(&s.LC[s.rows.rowC[c] + (ip >> 5)], ~(1u << (ip & 31)));  // t in c's c-role row
-    }
+    if (UNLINK) unlink_tri(s, t);
   }
   const uint32_t nn = in->nnext;
   for (uint64_t i = gtid; i < nn; i += gsize) {
     const uint32_t t = nxt[i];
     atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
+    if (UNLINK) mark_dying(s, t);
   }
   if (need || collect_round) {
     const int32_t take = collect_round ? L : L + 1;  // also the k of the collected triangles
@@ -649,6 +781,9 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
           set |= kCur << (2 * f);
         }
         atomic_or_u32(&p.st[i], set);  // 00 -> 01
+        if (UNLINK) {
+          for (uint32_t d = die; d; d &= d - 1u) mark_dying(s, static_cast<uint32_t>(16 * i + __builtin_ctz(d)));
+        }
       }
       __syncthreads();
     }
@@ -748,9 +883,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
-    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, lds[w], [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3]) {
-      destroy_k4(tt, q, p, level, nxt, ncount);
-    });
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, lds[w],
+                         [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3], uint32_t die) {
+                           destroy_k4_live(tt, q, die, p, level, nxt, ncount);
+                         });
   }
   prepare_out_slot(out, done);
 }
@@ -773,12 +909,14 @@ __global__ __launch_bounds__(kBlock) void k_residual_sum(const int32_t* __restri
   }
 }
 
-__global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC) {
+__global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* __restrict__ IO,
+                            const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < WA; i += stride)
-    s.LA[i] = s.rows.Aout[i] | s.rows.Ain[i].bits;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < WC; i += stride)
-    s.LC[i] = s.rows.Cm[i].bits;
+  const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t i = tid; i < WA; i += stride) s.LA[i] = LiveWord{s.rows.Aout[i] | s.rows.Ain[i].bits, 0u};
+  for (uint64_t i = tid; i < WC; i += stride) s.LC[i] = LiveWord{s.rows.Cm[i].bits, 0u};
+  for (uint64_t u = tid; u < n; u += stride)  // in-lists are sorted by source id
+    icut[u] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(IE, IO[u], IO[u + 1], static_cast<uint32_t>(u)) - IO[u]);
 }
 
 }  // namespace
@@ -804,9 +942,17 @@ void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsi
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t* k4, hipStream_t st) {
-  if (!s.T) return;
-  hipLaunchKernelGGL(k_k4_owned, dim3(grid_for(s.T, kBlock, 1u << 14)), dim3(kBlock), 0, st, s, k4off, k4);
+void launch_k4_owned_bounds(const uint64_t* k4off, uint64_t T, uint64_t first, uint64_t count, uint64_t* out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_k4_owned_bounds, dim3(1), dim3(64), 0, st, k4off, T, first, count, out);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t t_begin, uint64_t t_end, uint64_t first,
+                     uint64_t count, uint64_t* k4, hipStream_t st) {
+  if (t_end <= t_begin || !count) return;
+  hipLaunchKernelGGL(k_k4_owned, dim3(grid_for(t_end - t_begin, kBlock, 1u << 14)), dim3(kBlock), 0, st, s, k4off,
+                     t_begin, t_end, first, count, k4);
   NUC34_LAUNCH_CHECK();
 }
 
@@ -818,9 +964,11 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s) {
-  if (!WA && !WC) return;
-  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(WA > WC ? WA : WC, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC);
+void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
+                      uint32_t* icut, hipStream_t s) {
+  const uint64_t most = WA > WC ? (WA > n ? WA : n) : (WC > n ? WC : n);
+  if (!most) return;
+  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC, IO, IE, n, icut);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -19,12 +19,24 @@ struct K4Src {
   const uint32_t* cIdx;   // triangle id per c-slot
   const uint32_t* ainpos; // per triangle: inpos of its a-edge
   uint64_t T;
-  // Live rows for the on-demand peel (null elsewhere): LA = (Aout|Ain) and
-  // LC = Cm bits, each triangle's three bits cleared after the round it dies
-  // in, so a dying triangle enumerates only K4s whose two partners on its
-  // lowest vertex's rows are still alive or dying in the same round.
-  uint32_t* LA;
-  uint32_t* LC;
+  // Live rows for the on-demand peel (null elsewhere), word for word beside
+  // (Aout|Ain) and Cm: `live` holds the bits of the triangles not yet retired,
+  // `dying` those of the current round's frontier. A triangle's three bits of
+  // both planes are cleared after the round it dies in, so a dying triangle
+  // enumerates only K4s whose two partners on its lowest vertex's rows are
+  // still alive or dying in the same round, and it drops at once, before any
+  // partner lookup, every K4 one of those two partners owns (dying now with a
+  // lower index).
+  struct LiveWord* LA;
+  struct LiveWord* LC;
+  // per vertex u: in-neighbours with a lower vertex id than u (a prefix of
+  // N-(u), whose triangles precede u's in the canonical order); on-demand peel only
+  const uint32_t* icut;
+};
+
+struct alignas(8) LiveWord {
+  uint32_t live;
+  uint32_t dying;
 };
 
 // sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
@@ -36,8 +48,14 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
 // in that vertex's out-list, then the others in order of their fourth vertex's
 // position in the lowest vertex's in-list.
 void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st);
-// Owned K4s: (t << 32 | position of the fourth vertex in N+(u)) at k4off[t] ..
-void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t* k4owned, hipStream_t st);
+// Owned K4s: (t << 32 | position of the fourth vertex in N+(u)); the records
+// [first, first + count) of the list laid out at k4off[t] .. (k4off = exclusive
+// scan of own) go to k4owned[0, count), walking triangles [t_begin, t_end).
+void launch_k4_owned(const K4Src& s, const uint64_t* k4off, uint64_t t_begin, uint64_t t_end, uint64_t first,
+                     uint64_t count, uint64_t* k4owned, hipStream_t st);
+// out[0..1] = the triangles holding owned records first and first + count - 1.
+void launch_k4_owned_bounds(const uint64_t* k4off, uint64_t T, uint64_t first, uint64_t count, uint64_t* out,
+                            hipStream_t st);
 // The K4 list in canonical order (by the triangle on the three lowest vertices,
 // then by fourth vertex) as {T(pqs), T(pqr), T(prs), T(qrs)} for p<q<r<s, for
 // owned-K4 records [first, first + count).
@@ -105,7 +123,9 @@ void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
 // Sum of the residual supports after the peel into *total (zeroed by the
 // caller): the peel applied 4Q minus this many support decrements.
 void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s);
-// LA[i] = Aout[i] | Ain[i].bits for i < WA, LC[i] = Cm[i].bits for i < WC.
-void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, hipStream_t s);
+// LA[i] = {Aout[i] | Ain[i].bits, 0} for i < WA, LC[i] = {Cm[i].bits, 0} for
+// i < WC, and icut for the n vertices of the in-CSR (IO, IE).
+void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
+                      uint32_t* icut, hipStream_t s);
 
 }  // namespace nuc34

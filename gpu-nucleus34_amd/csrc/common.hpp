@@ -102,6 +102,9 @@ __device__ __forceinline__ uint32_t atomic_or_u32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ uint32_t atomic_and_u32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint64_t atomic_and_u64(uint64_t* p, uint64_t v) {
+  return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ uint32_t atomic_xor_u32(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

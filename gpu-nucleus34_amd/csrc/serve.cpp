@@ -1,8 +1,10 @@
 // Serving loop over the C ABI: a queue of independent graphs per job class
 // (small jobs, batched as disjoint unions; large jobs, one at a time) and
-// worker threads with their own device contexts and streams (one for small
-// jobs, `large_workers` for large ones), so a large job never holds small ones
-// back, and one large job's host work overlaps another's device work.
+// worker threads with their own device contexts and streams (`small_workers`
+// for small jobs, `large_workers` for large ones), so a large job never holds
+// small ones back, one job's host work overlaps another's device work, and a
+// small job that arrives while a batch runs starts on the next free small
+// worker at once (continuous batching) rather than after that batch.
 //
 // The reference runs one graph per process (nucleus/cuNucleus34.cu:147-175);
 // its memory plans (nucleus/utils.cuh:1329-1340, nucleus/cuNucleus34.cu:563-585)
@@ -247,7 +249,9 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
   if (!c.max_batch_edges) c.max_batch_edges = 256ull << 20;
   if (!c.max_batch_graphs) c.max_batch_graphs = 4096;
   if (!c.large_workers) c.large_workers = 2;
-  if (c.large_workers > 64) return NUC34_E_INVALID;  // each worker holds a context, a stream and a thread
+  if (!c.small_workers) c.small_workers = 2;
+  // each worker holds a context, a stream and a thread
+  if (c.large_workers > 64 || c.small_workers > 64) return NUC34_E_INVALID;
   if (!c.hbm_budget) {
     char buf[1024];
     if (nuc34_device_info(c.device, buf, sizeof(buf)) != NUC34_OK) return NUC34_E_HIP;
@@ -256,11 +260,11 @@ int nuc34_serve_create(const nuc34_serve_config* cfg, nuc34_server** out) {
     c.hbm_budget = static_cast<uint64_t>(0.9 * static_cast<double>(std::strtoull(f + 18, nullptr, 10)));
   }
   const uint64_t big = static_cast<uint64_t>(c.large_fraction * static_cast<double>(c.hbm_budget));
-  s->workers.resize(1 + c.large_workers);
+  s->workers.resize(c.small_workers + c.large_workers);
   for (size_t i = 0; i < s->workers.size(); ++i) {
     Worker& w = s->workers[i];
-    w.cls = i == 0 ? 0 : 1;
-    const uint64_t budget = i == 0 ? c.hbm_budget - big : big / c.large_workers;
+    w.cls = i < c.small_workers ? 0 : 1;
+    const uint64_t budget = w.cls == 0 ? (c.hbm_budget - big) / c.small_workers : big / c.large_workers;
     const int rc = nuc34_ctx_create(c.device, budget, &w.ctx);
     if (rc != NUC34_OK) {
       for (size_t k = 0; k < i; ++k) nuc34_ctx_destroy(s->workers[k].ctx);

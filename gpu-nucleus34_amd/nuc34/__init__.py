@@ -33,7 +33,7 @@ class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("large_fraction", C.c_double),
                 ("large_edges", C.c_uint64), ("max_batch_edges", C.c_uint64), ("max_batch_graphs", C.c_uint32),
                 ("variant", C.c_int), ("batch_window_ms", C.c_double), ("large_workers", C.c_uint32),
-                ("large_wedges", C.c_uint64)]
+                ("large_wedges", C.c_uint64), ("small_workers", C.c_uint32)]
 
 
 class JobInfo(C.Structure):
@@ -114,6 +114,7 @@ def _load() -> C.CDLL:
         "nuc34_copy_oriented": (C.c_int, [vp, u32p, u32p]),
         "nuc34_triangles": (C.c_int, [vp, C.c_uint32, u64p]),
         "nuc34_copy_triangles": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p]),
+        "nuc34_copy_edge_triangles": (C.c_int, [vp, u32p]),
         "nuc34_support": (C.c_int, [vp, u64p]),
         "nuc34_copy_support": (C.c_int, [vp, i32p]),
         "nuc34_decompose": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(Stats)]),
@@ -352,6 +353,14 @@ class Context:
         self._check(_lib.nuc34_copy_triangles(self._h, first, count, _u32(out)))
         return out[:count]
 
+    def copy_edge_triangles(self) -> np.ndarray:
+        """Triangles on each oriented edge (m entries, any role)."""
+        self.triangles()
+        _, m = self.dims()
+        out = np.empty(max(m, 1), dtype=np.uint32)
+        self._check(_lib.nuc34_copy_edge_triangles(self._h, _u32(out)))
+        return out[:m]
+
     def support(self) -> int:
         Q = C.c_uint64()
         self._check(_lib.nuc34_support(self._h, C.byref(Q)))
@@ -423,15 +432,16 @@ def decompose_graph(g: HostGraph, variant="auto", parts: int = 1, device: int = 
 class Server:
     """Decomposition service (nuc34_serve_*): submit undirected graphs from any
     thread, wait for and take their results. Small jobs are batched as disjoint
-    unions by one worker, large ones run alone on `large_workers` workers; the
+    unions by `small_workers` workers (a job arriving while a batch runs starts
+    on the next free one), large ones run alone on `large_workers` workers; the
     workers share the device through their own contexts and streams."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, large_fraction: float = 0.0, large_edges: int = 0,
                  max_batch_edges: int = 0, max_batch_graphs: int = 0, variant="auto", batch_window_ms: float = 0.0,
-                 large_workers: int = 0, large_wedges: int = 0):
+                 large_workers: int = 0, large_wedges: int = 0, small_workers: int = 0):
         cfg = ServeConfig(device, hbm_budget, large_fraction, large_edges, max_batch_edges, max_batch_graphs,
                           VARIANTS[variant] if isinstance(variant, str) else int(variant), batch_window_ms,
-                          large_workers, large_wedges)
+                          large_workers, large_wedges, small_workers)
         self._h = C.c_void_p()
         rc = _lib.nuc34_serve_create(C.byref(cfg), C.byref(self._h))
         if rc != OK:

@@ -154,6 +154,11 @@ int nuc34_triangles(nuc34_ctx* ctx, uint32_t parts, uint64_t* T);
 /* abc: 3 * count entries (a, b, c) for triangles [first, first + count). */
 int nuc34_copy_triangles(nuc34_ctx* ctx, uint64_t first, uint64_t count, uint32_t* abc);
 
+/* counts: m entries, the number of triangles on each oriented edge (any
+ * role). It replaces the per-edge extents of the on-demand edge->triangle
+ * incidence CSR, incO[e+1] - incO[e] (nucleus/cuNucleus34_ondemand.cu:502-555). */
+int nuc34_copy_edge_triangles(nuc34_ctx* ctx, uint32_t* counts);
+
 /* Edge->triangle incidence and per-triangle K4 support; Q = sum / 4. */
 int nuc34_support(nuc34_ctx* ctx, uint64_t* Q);
 int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
@@ -220,12 +225,15 @@ int nuc34_device_info(int device, char* buf, size_t len);
  * No counterpart in the reference (one graph per process): a decomposition
  * service for a stream of independent graphs. Any thread submits jobs; worker
  * threads, each with its own context and stream on the device, take them in
- * arrival order: one small-job lane and `large_workers` large-job workers. The small-job lane packs every queued job that fits its
- * caps into one disjoint-union batch (one pass of every kernel, shared peel
- * rounds) as soon as it is free; the large-job lane runs graphs at or above
- * `large_edges` edge entries or `large_wedges` wedges one at a time, so they
- * never hold small jobs back. Admission
- * is by HBM footprint: each lane holds its share of the budget, a batch runs
+ * arrival order: `small_workers` small-job workers and `large_workers`
+ * large-job workers. A free small-job worker packs every queued job that fits
+ * its caps into one disjoint-union batch (one pass of every kernel, shared peel
+ * rounds); with more than one, jobs that arrive while a batch is on the device
+ * start at once on another worker, their work overlapping the running batch
+ * (continuous batching), instead of waiting for the batch's slowest graph. The
+ * large-job lane runs graphs at or above `large_edges` edge entries or
+ * `large_wedges` wedges one at a time, so they never hold small jobs back.
+ * Admission is by HBM footprint: each lane holds its share of the budget, a batch runs
  * in the configured variant (NUC34_AUTO: on demand, O(T) memory), and a batch
  * that runs out of its lane's budget is split in halves and retried.
  */
@@ -245,6 +253,8 @@ typedef struct nuc34_serve_config {
   uint64_t large_wedges;     /* wedges, sum over vertices of d(d-1)/2, at which a job is large
                                 whatever its edge count (0 = 2^28): a graph with hubs costs more
                                 than its edge count says */
+  uint32_t small_workers;    /* workers (contexts, streams) for small-job batches (0 = 2, at most 64);
+                                they share the small lane's budget equally */
 } nuc34_serve_config;
 
 typedef struct nuc34_job_info {

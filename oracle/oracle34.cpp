@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iterator>
 #include <limits>
@@ -430,6 +432,13 @@ Result host_path(const Graph& g, size_t parts) {
 }
 
 Result ondemand(const Graph& g, size_t parts) {
+  // ORACLE34_PROGRESS set: one stderr line per phase and every 256 peel rounds
+  // (long runs at full size stay visibly alive)
+  const bool progress = std::getenv("ORACLE34_PROGRESS") != nullptr;
+  auto note = [&](const char* what, uint64_t a, uint64_t b) {
+    if (progress) std::fprintf(stderr, "[oracle34 ondemand] %s %llu %llu\n", what, static_cast<unsigned long long>(a),
+                               static_cast<unsigned long long>(b));
+  };
   Result r;
   const size_t P = parts ? parts : static_cast<size_t>(omp_get_num_procs());
   auto t0 = clk::now();
@@ -439,6 +448,7 @@ Result ondemand(const Graph& g, size_t parts) {
   r.t_tri = secs(t0, t1);
   const size_t T = r.tris.size();
   r.T = T;
+  note("triangles", T, 0);
   if (!T) return r;
   const size_t m = g.E.size();
   omp_set_num_threads(static_cast<int>(P));
@@ -470,6 +480,7 @@ Result ondemand(const Graph& g, size_t parts) {
   for (size_t e = 0; e < m; ++e) std::sort(inc.begin() + incO[e], inc.begin() + incO[e + 1]);
   auto t2 = clk::now();
   r.t_incidence = secs(t1, t2);
+  note("incidence", 3 * T, 0);
   // 3-way zipper over the lists of edges b, a, c (for_each_clique_of, 136-170).
   auto for_each_clique = [&](size_t ti, auto&& fn) {
     const Tri& t = r.tris[ti];
@@ -506,6 +517,7 @@ Result ondemand(const Graph& g, size_t parts) {
   r.t_support = secs(t2, t3);
   r.support = DD;
   r.Q = total / 4;
+  note("support", r.Q, 0);
   if (!r.Q) {
     r.k.assign(T, 0);
     return r;
@@ -570,6 +582,7 @@ Result ondemand(const Graph& g, size_t parts) {
     for (auto& l : local) crossed.insert(crossed.end(), l.begin(), l.end());
     peeled += dying.size();
     ++round;
+    if (round % 256 == 0) note("peel round / peeled", round, peeled);
   }
   auto t4 = clk::now();
   r.t_peel = secs(t3, t4);

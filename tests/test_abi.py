@@ -43,7 +43,7 @@ def test_struct_layouts_match_header(tmp_path):
     import ctypes
     import subprocess
     import nuc34
-    fields = {"nuc34_serve_config": (nuc34.ServeConfig, ["hbm_budget", "large_fraction", "variant", "batch_window_ms", "large_workers", "large_wedges"]),
+    fields = {"nuc34_serve_config": (nuc34.ServeConfig, ["hbm_budget", "large_fraction", "variant", "batch_window_ms", "large_workers", "large_wedges", "small_workers"]),
               "nuc34_job_info": (nuc34.JobInfo, ["n", "t_submit", "t_done", "error"]),
               "nuc34_serve_stats": (nuc34.ServeStats, ["splits", "busy_s"]),
               "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels", "debits"]),

@@ -153,3 +153,125 @@ def test_serve_small_jobs_run_beside_a_large_one():
     assert inside
     t1, k1, _ = single(big)
     assert np.array_equal(tb, t1) and np.array_equal(kb, k1)
+
+
+def test_serve_small_jobs_overtake_a_long_small_batch():
+    """Continuous batching inside the small-job lane: with two small-job
+    workers, small jobs that arrive while a long small-lane batch is on the
+    device start on the other worker and finish before that batch ends; with
+    one worker they wait for it. Every result, alone or in the mix, equals the
+    graph's own decomposition."""
+    import time
+    big = nuc34.gen_preset("rmat16")  # lane thresholds raised below, so it is a small-lane batch of one
+    pool = [nuc34.gen_batch_graph(4, i) for i in range(0, 24, 3)]
+    for workers in (2, 1):
+        print(f"[serve] long small-lane batch, {workers} small-job worker(s)", flush=True)
+        with nuc34.Server(hbm_budget=32 << 30, large_edges=1 << 40, large_wedges=1 << 62,
+                          small_workers=workers) as srv:
+            jb = srv.submit(big)
+            time.sleep(0.005)  # the big job is taken alone
+            jobs, gs = [], []
+            while len(jobs) < 40:
+                g = pool[len(jobs) % len(pool)]
+                jobs.append(srv.submit(g))
+                gs.append(g)
+                if srv.wait(jb, 0) is not None:
+                    break
+                time.sleep(0.002)
+            ib = srv.wait(jb, 120000)
+            infos = check_jobs(srv, jobs, gs)
+            assert ib is not None and ib["status"] == nuc34.OK, ib
+            tb, kb = srv.take(jb, ib["triangles"])
+        assert ib["batch_graphs"] == 1
+        inside = [i for i in infos if ib["t_start"] <= i["t_start"] and i["t_done"] <= ib["t_done"]]
+        print(f"[serve] {len(inside)} of {len(infos)} small jobs started and finished inside the "
+              f"{1e3 * (ib['t_done'] - ib['t_start']):.0f} ms batch", flush=True)
+        if workers == 2:
+            assert inside and all(i["batch_id"] != ib["batch_id"] for i in infos)
+        else:
+            assert all(i["t_start"] >= ib["t_done"] for i in infos)
+        t1, k1, _ = single(big)
+        assert np.array_equal(tb, t1) and np.array_equal(kb, k1)
+
+
+def test_serve_poisson_stream_with_rmat18_jobs():
+    """The serving workload: a Poisson stream (200 jobs/s, seed 4, 1 s of
+    arrivals) of the many-graphs mix plus R-MAT scale 18 jobs at a 5% share,
+    under HBM admission with the variant chosen per batch from its footprint
+    (NUC34_AUTO_FIT: the scale-18 lists, 114 GB, do not fit a large worker's
+    share of the budget; small batches' do). Every R-MAT scale-18 result
+    hashes to the golden raw k and triangle arrays, every small job equals its
+    own decomposition, and the smallest ones equal the oracle's."""
+    import hashlib
+    import json
+    import os
+    import queue
+    import time
+    from conftest import GOLDEN
+    gold = json.load(open(os.path.join(GOLDEN, "hashes.json")))["rmat18"]
+    rng = np.random.default_rng(4)
+    pool = [nuc34.gen_batch_graph(4, i) for i in range(64)]
+    large = nuc34.gen_preset("rmat18")
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1 / 200.0)
+        if t > 1.0:
+            break
+        big = bool(rng.random() < 0.05)
+        arrivals.append((t, big, int(rng.integers(len(pool)))))
+    n_large = sum(b for _, b, _ in arrivals)
+    print(f"[serve] Poisson stream: {len(arrivals)} jobs, {n_large} R-MAT s18", flush=True)
+    assert n_large >= 5
+    alone = {}
+    for i in sorted({i for _, b, i in arrivals if not b}):
+        t1, k1, _ = single(pool[i])
+        alone[i] = (t1, k1)
+    bad, done, variants = [], [], set()
+    q = queue.Queue()
+    with nuc34.Server(0, variant="auto_fit") as srv:
+        import threading
+
+        def collect():
+            while True:
+                item = q.get()
+                if item is None:
+                    return
+                job, big, i = item
+                info = srv.wait(job, 600000)
+                if info is None or info["status"] != nuc34.OK:
+                    bad.append((big, i, info))
+                    continue
+                tris, k = srv.take(job, info["triangles"])
+                variants.add((big, info["variant"]))
+                if big:
+                    ok = (hashlib.sha256(k.tobytes()).hexdigest() == gold["k_raw_sha256"] and
+                          hashlib.sha256(tris.tobytes()).hexdigest() == gold["tri_raw_sha256"] and
+                          info["k_max"] == gold["K_max"] and info["four_cliques"] == gold["Q"])
+                    print(f"[serve] R-MAT s18 job done, {1e3 * (info['t_done'] - info['t_submit']):.0f} ms, "
+                          f"bit-exact={ok}", flush=True)
+                else:
+                    ok = np.array_equal(tris, alone[i][0]) and np.array_equal(k, alone[i][1])
+                if not ok:
+                    bad.append((big, i, "mismatch"))
+                done.append((big, info["t_done"] - info["t_submit"]))
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = srv.now()
+        for ta, big, i in arrivals:
+            d = t0 + ta - srv.now()
+            if d > 0:
+                time.sleep(d)
+            q.put((srv.submit(large if big else pool[i]), big, i))
+        q.put(None)
+        th.join()
+        st = srv.stats()
+    assert not bad, bad[:5]
+    assert len(done) == len(arrivals) and st["completed"] == len(arrivals) and st["failed"] == 0
+    assert (True, nuc34.ONDEMAND) in variants and (False, nuc34.DIRECT) in variants
+    lat = np.array([d for b, d in done if not b])
+    print(f"[serve] small p50/p99 {1e3 * np.percentile(lat, 50):.1f}/{1e3 * np.percentile(lat, 99):.1f} ms", flush=True)
+    order = sorted(alone, key=lambda i: pool[i].m)
+    for i in order[:2]:
+        ref = oracle34.OracleGraph.from_csr(pool[i].O, pool[i].E, orient=True).run("host", 4)
+        assert np.array_equal(alone[i][1], ref["k"]) and np.array_equal(alone[i][0], ref["tris"])
