@@ -356,7 +356,8 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
   const uint64_t m = c->m;
   const uint32_t n = c->n;
   const auto ranges = edge_parts(c, parts);
-  c->st.ms_triangles = timed(c, [&] {
+  // enumeration: in-CSR, row layout, the bits pass
+  const double ms_enum = timed(c, [&] {
     // in-CSR: the c-role rows are indexed by position in the source's in-list
     c->IO = c->alloc.get((static_cast<uint64_t>(n) + 1) * 4, "in-offsets");
     c->IE = c->alloc.get(m * 4 + 4, "in-lists");
@@ -390,6 +391,10 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
         launch_tri_bits(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
                         c->inpos.as<uint32_t>(), tri_tile(c), c->s);
     });
+  });
+  // canonical order: role counts, ids and ranks, then the records at their ids
+  c->st.ms_tri_order = timed(c, [&] {
+    const BitRows rows = bitrows(c);
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
       // a-role counts give each edge's first triangle id, c-role counts its
@@ -420,6 +425,7 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
                         c->cIdx.as<uint32_t>(), c->ainpos.as<uint32_t>(), tri_tile(c), c->s);
     });
   });
+  c->st.ms_triangles = ms_enum + c->st.ms_tri_order;
   c->has_tris = true;
   c->st.triangles = c->T;
   c->st.incidence_entries = 3 * c->T;
@@ -481,6 +487,7 @@ void run_peel(nuc34_ctx* c, int variant) {
   c->st.rounds = 0;
   c->st.k_max = 0;
   c->st.unpeeled = 0;
+  c->st.ms_peel = c->st.ms_peel_init = 0;
   if (!T) return;
   if (!c->Q) {
     zero(c, c->kv);
@@ -519,10 +526,12 @@ void run_peel(nuc34_ctx* c, int variant) {
   };
   PeelCtl hs[2] = {};
   PeelCtl h{};
-  c->st.ms_peel = timed(c, [&] {
+  c->st.ms_peel_init = timed(c, [&] {
     launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
     if (variant != NUC34_DIRECT)
       launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(), c->s);
+  });
+  c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
     struct Captured {  // destroyed on every way out, errors included
       hipStream_t s = nullptr;
       hipGraph_t graph = nullptr;

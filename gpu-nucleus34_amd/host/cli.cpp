@@ -157,13 +157,36 @@ int main(int argc, char** argv) {
 #if DIRECT
     js.field("data_memory_bytes", (static_cast<uint64_t>(n) + 1) * 4 + mu * 4);
 #endif
+    {
+      std::ostringstream os;
+      os << "vertices=" << n << " oriented_edges=" << m << " host_graph=" << (((static_cast<uint64_t>(n) + 1) * 4 + mu * 4) >> 20)
+         << " MiB load=" << std::fixed << std::setprecision(1) << t_data << "s";
+      nuc34cli::mem_note("graph", os.str());
+    }
+#if DIRECT
+    {
+      // no path-count scratch: the intersection writes bit-packed rows of
+      // ceil(d+/32) and ceil(d-/32) words per edge
+      std::ostringstream os;
+      os << "parts=" << parts << " mode=bit-rows (no path scratch)";
+      nuc34cli::mem_note("tri-plan", os.str());
+    }
+#endif
     set_step("1.1 Triangle counting");
     uint64_t T = 0;
     check(nuc34_triangles(ctx, parts, &T), ctx);
     nuc34_stats st{};
     nuc34_get_stats(ctx, &st);
-    js.field("triangle_time_sec", st.ms_triangles / 1e3);
+    // enumeration into the bit rows; the canonical ordering is the reference's
+    // triangle graph (direct) or incidence build (on demand)
+    js.field("triangle_time_sec", (st.ms_triangles - st.ms_tri_order) / 1e3);
     js.field("triangle_count", T);
+    {
+      std::ostringstream os;
+      os << "triangles=" << T << " device_tris=" << ((T * 12) >> 20) << " MiB time=" << std::fixed
+         << std::setprecision(1) << st.ms_triangles / 1e3 << "s";
+      nuc34cli::mem_note("tri-done", os.str());
+    }
     std::vector<uint32_t> first(3 * std::min<uint64_t>(kPrintCap, T));
     check(nuc34_copy_triangles(ctx, 0, first.size() / 3, first.data()), ctx);
 #if DIRECT
@@ -182,16 +205,21 @@ int main(int argc, char** argv) {
     check(nuc34_support(ctx, &Q), ctx);
     nuc34_get_stats(ctx, &st);
 #if DIRECT
-    js.field("triangle_graph_time_sec", st.ms_incidence / 1e3);
+    js.field("triangle_graph_time_sec", st.ms_tri_order / 1e3);
     js.field("four_clique_time_sec", st.ms_support / 1e3);
     js.field("four_clique_count", Q);
 #else
-    js.field("incidence_time_sec", st.ms_incidence / 1e3);
+    js.field("incidence_time_sec", st.ms_tri_order / 1e3);
     js.field("incidence_entries", st.incidence_entries);
     js.field("four_clique_time_sec", st.ms_support / 1e3);
     js.field("four_clique_count", Q);
 #endif
-    std::cerr << "[mem] support four_cliques=" << Q << " incidences=" << 4 * Q << std::endl;
+    {
+      std::ostringstream os;
+      os << "four_cliques=" << Q << " peel_resident=" << ((T * 24 + (DIRECT ? 48 * Q + 8 * (T + 1) : 0)) >> 20)
+         << " MiB time=" << std::fixed << std::setprecision(1) << st.ms_support / 1e3 << "s";
+      nuc34cli::mem_note("fc-done", os.str());
+    }
     if (!Q) {
 #if DIRECT
       js.array("four-cliques", first.begin(), first.begin(), [](std::ostream&, uint32_t) {});
@@ -209,15 +237,17 @@ int main(int argc, char** argv) {
       for (size_t i = 0; i < q.size() / 4; ++i) rows.push_back(nuc34cli::tuple4(&q[4 * i]));
       js.array("four-cliques", rows.begin(), rows.end(), [](std::ostream& os, const std::string& r) { os << '"' << r << '"'; });
     }
+    nuc34cli::mem_note("inc-done", "triangle->four-clique lists built");
     js.field("four_clique_graph_time_sec", st.ms_k4_list / 1e3);
     js.field("peeling_mode", "resident-direct");
-    js.field("peeling_preprocess_time_sec", 0.0);
+    js.field("peeling_preprocess_time_sec", st.ms_peel_init / 1e3);
+    js.field("peeling_time_sec", (st.ms_peel - st.ms_peel_init) / 1e3);
 #else
     js.field("unpeeled_count", st.unpeeled);
     if (st.unpeeled) std::cerr << "[warn] " << st.unpeeled << " triangles left unpeeled" << std::endl;
     js.field("peeling_mode", "ondemand-reenumerate");
-#endif
     js.field("peeling_time_sec", st.ms_peel / 1e3);
+#endif
     js.field("peeling_iterations", st.rounds);
     js.field("K_max", st.k_max);
     js.field("total_time_sec", st.ms_total / 1e3);

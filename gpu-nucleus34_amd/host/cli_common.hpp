@@ -74,6 +74,18 @@ inline std::vector<std::pair<std::string, std::string>> device_props(int dev) {
   return out;
 }
 
+// "[mem] <tag> <detail> | gpu_free=<free>/<total> MiB" on stderr: the device
+// memory milestones of the reference's CLI (nucleus/cuNucleus34.cu:72-79).
+inline void mem_note(const char* tag, const std::string& detail) {
+  unsigned long long free_b = 0, total_b = 0;
+  for (const auto& p : device_props(0)) {
+    if (p.first == "free_memory_bytes") free_b = std::stoull(p.second);
+    if (p.first == "global_memory_bytes") total_b = std::stoull(p.second);
+  }
+  std::fprintf(stderr, "[mem] %s %s | gpu_free=%llu/%llu MiB\n", tag, detail.c_str(), free_b >> 20, total_b >> 20);
+  std::fflush(stderr);
+}
+
 inline std::string tuple3(const uint32_t* t) {
   std::ostringstream os;
   os << '(' << t[0] << ',' << t[1] << ',' << t[2] << ')';

@@ -79,7 +79,7 @@ class Stats(C.Structure):
         ("ms_orient", C.c_double), ("ms_triangles", C.c_double), ("ms_incidence", C.c_double),
         ("ms_support", C.c_double), ("ms_k4_list", C.c_double), ("ms_peel", C.c_double),
         ("ms_total", C.c_double), ("n_kernels", C.c_uint32), ("kernels", KernelStat * 8),
-        ("debits", C.c_uint64),
+        ("debits", C.c_uint64), ("ms_tri_order", C.c_double), ("ms_peel_init", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -80,6 +80,9 @@ typedef struct nuc34_stats {
   uint32_t n_kernels;
   nuc34_kernel_stat kernels[NUC34_MAX_KERNEL_STATS];
   uint64_t debits;            /* support decrements the peel applied (device atomics) */
+  double ms_tri_order;        /* part of ms_triangles: canonical (a,b) ids, ranked incidence rows and
+                                 the triangle records (the rest is the enumeration into the bit rows) */
+  double ms_peel_init;        /* part of ms_peel: supports, states and live rows set up before round 0 */
 } nuc34_stats;
 
 /* ---- host-side datasets (reference loaders, exact semantics) ------------ */

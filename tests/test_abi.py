@@ -46,7 +46,8 @@ def test_struct_layouts_match_header(tmp_path):
     fields = {"nuc34_serve_config": (nuc34.ServeConfig, ["hbm_budget", "large_fraction", "variant", "batch_window_ms", "large_workers", "large_wedges", "small_workers"]),
               "nuc34_job_info": (nuc34.JobInfo, ["n", "t_submit", "t_done", "error"]),
               "nuc34_serve_stats": (nuc34.ServeStats, ["splits", "busy_s"]),
-              "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels", "debits"]),
+              "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels", "debits", "ms_tri_order",
+                                             "ms_peel_init"]),
               "nuc34_tuning": (nuc34.Tuning, ["peel_tile_cap", "k4_list_grid_cap", "tri_tile_edges",
                                                 "peel_finish_grid"]),
               "nuc34_device_rates": (nuc34.DeviceRates, ["atomic_gops", "load_hbm_gops"])}
