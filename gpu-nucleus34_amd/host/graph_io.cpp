@@ -281,6 +281,11 @@ LoadedGraph read_grh(const std::string& path) {
   if (!f) throw std::runtime_error("Truncated .grh header.");
   if (nV > std::numeric_limits<uint32_t>::max() || nE > std::numeric_limits<uint32_t>::max())
     throw std::runtime_error("Edges size are larger than defined type.");
+  // the body must be in the file before anything is sized from the header
+  f.seekg(0, std::ios::end);
+  const uint64_t file_bytes = static_cast<uint64_t>(f.tellg());
+  if (file_bytes < 24 + 4 * (nV + nE)) throw std::runtime_error("Truncated .grh body.");
+  f.seekg(24, std::ios::beg);
   LoadedGraph out;
   out.oriented = true;
   Csr& g = out.g;
@@ -291,8 +296,13 @@ LoadedGraph read_grh(const std::string& path) {
   f.read(reinterpret_cast<char*>(g.E.data()), static_cast<std::streamsize>(nE * 4));
   if (!f) throw std::runtime_error("Truncated .grh body.");
   g.O.assign(nV + 1, 0);
-  for (size_t v = 0; v < nV; ++v) g.O[v + 1] = g.O[v] + D[v];
-  if (g.O[nV] != nE) throw std::runtime_error("Degree sum of .grh does not match its edge count.");
+  uint64_t run = 0;  // in 64 bits: degrees that wrap a 32-bit sum are rejected, not folded
+  for (size_t v = 0; v < nV; ++v) {
+    run += D[v];
+    if (run > nE) throw std::runtime_error("Degree sum of .grh does not match its edge count.");
+    g.O[v + 1] = static_cast<uint32_t>(run);
+  }
+  if (run != nE) throw std::runtime_error("Degree sum of .grh does not match its edge count.");
   out.lines = nE;
   return out;
 }

@@ -13,6 +13,7 @@
 #include <iterator>
 #include <limits>
 #include <numeric>
+#include <parallel/algorithm>
 #include <sstream>
 #include <stdexcept>
 
@@ -190,7 +191,9 @@ std::vector<Tri> triangles_par(const Graph& g, const std::vector<size_t>& sp) {
 }
 
 TriGraph::TriGraph(std::vector<Tri>& tris, size_t m) {
-  std::sort(tris.begin(), tris.end(), [](const Tri& x, const Tri& y) {
+  // (a,b) keys are unique, so any correct sort gives the same order; the
+  // parallel one stands for the reference's std::execution::par sort
+  __gnu_parallel::sort(tris.begin(), tris.end(), [](const Tri& x, const Tri& y) {
     return x[0] != y[0] ? x[0] < y[0] : x[1] < y[1];
   });
   D.assign(m, 0);
@@ -506,12 +509,15 @@ Result ondemand(const Graph& g, size_t parts) {
   };
   std::vector<int32_t> DD(T);
   uint64_t total = 0;
+  std::atomic<uint64_t> counted{0};
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : total)
   for (size_t t = 0; t < T; ++t) {
     int32_t c = 0;
     for_each_clique(t, [&](uint32_t, uint32_t, uint32_t) { ++c; });
     DD[t] = c;
     total += static_cast<uint64_t>(c);
+    if (progress && ((counted.fetch_add(1, std::memory_order_relaxed) + 1) & ((1u << 24) - 1)) == 0)
+      note("support triangles done", counted.load(), T);
   }
   auto t3 = clk::now();
   r.t_support = secs(t2, t3);
@@ -538,6 +544,7 @@ Result ondemand(const Graph& g, size_t parts) {
   uint32_t round = 0;
   const int nth = omp_get_max_threads();
   std::vector<std::vector<uint32_t>> local(static_cast<size_t>(nth));
+  auto last_note = clk::now();
   while (peeled < T) {
     if (crossed.empty()) {
       int32_t mn = std::numeric_limits<int32_t>::max();
@@ -582,7 +589,11 @@ Result ondemand(const Graph& g, size_t parts) {
     for (auto& l : local) crossed.insert(crossed.end(), l.begin(), l.end());
     peeled += dying.size();
     ++round;
-    if (round % 256 == 0) note("peel round / peeled", round, peeled);
+    const auto now = clk::now();
+    if (round % 256 == 0 || secs(last_note, now) > 20) {
+      note("peel round / peeled", round, peeled);
+      last_note = now;
+    }
   }
   auto t4 = clk::now();
   r.t_peel = secs(t3, t4);
