@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -29,6 +30,10 @@ struct nuc34_graph {
 
 struct nuc34_ctx {
   int device = 0;
+  // no-allocation mode (nuc34_decompose_noalloc): buffers come from the
+  // device's scratch array, nothing is timed, peel rounds are launched eagerly
+  // and their control is read back with synchronous copies
+  bool noalloc = false;
   hipStream_t s = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Allocator alloc;
@@ -110,6 +115,10 @@ int guard(nuc34_ctx* c, F&& f) {
 // Milliseconds between two events recorded around `f` on the context stream.
 template <typename F>
 double timed(nuc34_ctx* c, F&& f) {
+  if (c->noalloc) {  // no events: nothing may be created
+    f();
+    return 0.0;
+  }
   NUC34_HIP(hipEventRecord(c->ev0, c->s));
   f();
   NUC34_HIP(hipEventRecord(c->ev1, c->s));
@@ -145,7 +154,7 @@ void par_memcpy(char* dst, const char* src, size_t bytes) {
 // on the context stream runs first.
 void to_host(nuc34_ctx* c, void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
-  if (bytes <= (size_t(1) << 20)) {
+  if (bytes <= (size_t(1) << 20) || c->noalloc) {
     NUC34_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     return;
@@ -179,7 +188,7 @@ void to_device(nuc34_ctx* c, void* dst, const std::vector<std::pair<const void*,
   size_t total = 0;
   for (const auto& sg : segs) total += sg.second;
   if (!total) return;
-  if (total <= (size_t(1) << 20)) {
+  if (total <= (size_t(1) << 20) || c->noalloc) {
     size_t off = 0;
     for (const auto& sg : segs) {
       if (sg.second)
@@ -226,6 +235,10 @@ const char* const kKernelNames[K_COUNT] = {"tri_enum_bits", "tri_enum_emit", "su
 // Records events around `f` on the context stream; read back by ktime_collect.
 template <typename F>
 void ktime(nuc34_ctx* c, int id, uint64_t launches, F&& f) {
+  if (c->noalloc) {
+    f();
+    return;
+  }
   auto& k = c->kt[id];
   if (k.pending) {
     NUC34_HIP(hipEventSynchronize(k.b));
@@ -545,6 +558,21 @@ void run_peel(nuc34_ctx* c, int variant) {
       }
     } cap;
     cap.s = c->s;
+    const uint64_t max_launches = 2 * T / per_launch + 16;  // every second round fixes >= 1 triangle
+    if (c->noalloc) {
+      // rounds launched eagerly (instantiating a graph allocates); both
+      // control slots read back after every group
+      PeelCtl got[2];
+      for (uint64_t l = 0;; ++l) {
+        enqueue_rounds(per_launch);
+        NUC34_HIP(hipMemcpyAsync(got, ctlb.as<void>(), sizeof(got), hipMemcpyDeviceToHost, c->s));
+        NUC34_HIP(hipStreamSynchronize(c->s));
+        h = got[0].seq >= got[1].seq ? got[0] : got[1];
+        if (h.done) break;
+        if (l > max_launches) throw std::logic_error("peel did not finish");
+      }
+      return;
+    }
     if (!std::getenv("NUC34_NO_GRAPH") && hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess) {
       try {
         enqueue_rounds(per_launch);
@@ -563,7 +591,6 @@ void run_peel(nuc34_ctx* c, int variant) {
       (void)hipGetLastError();
     }
     const hipGraphExec_t exec = cap.exec;
-    const uint64_t max_launches = 2 * T / per_launch + 16;  // every second round fixes >= 1 triangle
     if (!c->ctl_host) {
       NUC34_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->ctl_host), 4 * sizeof(PeelCtl), hipHostMallocDefault));
       for (hipEvent_t& e : c->poll_ev) NUC34_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1345,3 +1372,113 @@ int nuc34_device_info(int device, char* buf, size_t len) {
 }
 
 }  // extern "C"
+
+// ---- decomposition without device allocation --------------------------------
+//
+// Every device buffer of nuc34_decompose_noalloc comes from `scratch`, a
+// file-scope device array: each device the library runs on holds its own copy
+// for the life of the process. Host code reaches a device's copy only through
+// the address hipGetSymbolAddress returns with that device current (the
+// array's name in host code is a host-side stand-in, not a device address).
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[NUC34_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+struct ScratchSlot {
+  std::mutex mu;  // one call at a time per device: the calls share the array
+  void* ptr = nullptr;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+}  // namespace
+
+extern "C" int nuc34_decompose_noalloc(int device, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m,
+                                       int variant, uint64_t tri_capacity, uint32_t* abc, int32_t* k,
+                                       nuc34_stats* stats) {
+  auto reject = [](int code, const std::string& msg) {
+    tls_create_error = msg;
+    return code;
+  };
+  if (device < 0 || device >= kMaxDevices) return reject(NUC34_E_INVALID, "device index out of range");
+  if (!O || (m && !E) || (tri_capacity && (!abc || !k))) return reject(NUC34_E_INVALID, "null argument");
+  if (variant != NUC34_DIRECT && variant != NUC34_ONDEMAND && variant != NUC34_AUTO)
+    return reject(NUC34_E_INVALID, "variant must be NUC34_DIRECT, NUC34_ONDEMAND or NUC34_AUTO");
+  if (O[0] != 0 || O[n] != m) return reject(NUC34_E_INVALID, "offsets do not span the edge array");
+  if (m % 2) return reject(NUC34_E_INVALID, "undirected CSR must hold both directions of every edge");
+  // the input alone (uploaded and oriented) takes 12 bytes per vertex and 10
+  // per edge entry; larger graphs cannot fit whatever their triangles
+  const uint64_t input_bytes = 12 * (static_cast<uint64_t>(n) + 1) + 10 * m;
+  if (input_bytes > NUC34_NOALLOC_SCRATCH_BYTES / 4)
+    return reject(NUC34_E_INVALID, "graph too large for the no-allocation scratch array (" +
+                                       std::to_string(input_bytes) + " bytes of input state, at most " +
+                                       std::to_string(NUC34_NOALLOC_SCRATCH_BYTES / 4) + ")");
+  ScratchSlot& slot = g_scratch[device];
+  std::lock_guard<std::mutex> lk(slot.mu);
+  if (hipSetDevice(device) != hipSuccess) {
+    (void)hipGetLastError();
+    return reject(NUC34_E_HIP, "hipSetDevice failed");
+  }
+  if (!slot.ptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return reject(NUC34_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    slot.ptr = p;
+  }
+  nuc34_ctx c;  // no stream (the null stream), no events, buffers from the array
+  c.device = device;
+  c.noalloc = true;
+  c.alloc.use_arena(slot.ptr, NUC34_NOALLOC_SCRATCH_BYTES);
+  const int rc = guard(&c, [&]() -> int {
+    {
+      DevBuf Ou = c.alloc.get((static_cast<uint64_t>(n) + 1) * 4, "input offsets");
+      DevBuf Eu = c.alloc.get(m * 4 + 4, "input edges");
+      to_device(&c, Ou.as<void>(), {{O, (static_cast<uint64_t>(n) + 1) * 4}});
+      if (m) to_device(&c, Eu.as<void>(), {{E, m * 4}});
+      DevBuf bad = c.alloc.get(4, "check flag");
+      launch_fill_u32(bad.as<uint32_t>(), 1, kNoVertex, c.s);
+      launch_check_csr(Ou.as<uint32_t>(), Eu.as<uint32_t>(), n, m, bad.as<uint32_t>(), c.s);
+      const uint32_t v = read_scalar<uint32_t>(&c, bad.as<uint32_t>());
+      if (v != kNoVertex)
+        return fail(&c, NUC34_E_INVALID, "vertex " + std::to_string(v) +
+                                             ": offsets out of order or range, id out of range, self-loop, or "
+                                             "list not strictly increasing");
+      bad.release();
+      install_graph(&c, Ou, Eu, n, m, false);
+    }
+    run_triangles(&c, 1);
+    if (c.T > tri_capacity) {
+      c.st.triangles = c.T;
+      if (stats) *stats = c.st;
+      return fail(&c, NUC34_E_INVALID, "triangle count " + std::to_string(c.T) + " exceeds tri_capacity " +
+                                           std::to_string(tri_capacity));
+    }
+    if (c.T) run_support(&c);
+    c.has_sup = true;
+    const int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
+    if (c.Q && v == NUC34_DIRECT) run_k4_lists(&c);
+    run_peel(&c, v);
+    c.st.variant = v;
+    c.st.peak_bytes = c.alloc.peak;
+    if (c.T) {
+      DevBuf buf = c.alloc.get(c.T * 12, "triangle copy");
+      launch_tri_abc(c.triA.as<uint32_t>(), c.triBC.as<uint64_t>(), c.T, buf.as<uint32_t>(), c.s);
+      to_host(&c, abc, buf.as<void>(), c.T * 12);
+      to_host(&c, k, c.kv.as<void>(), c.T * 4);
+    }
+    NUC34_HIP(hipStreamSynchronize(c.s));
+    if (stats) *stats = c.st;
+    return NUC34_OK;
+  });
+  if (rc != NUC34_OK) {
+    tls_create_error = c.err;
+    (void)hipStreamSynchronize(nullptr);  // nothing of this call may still run on the array
+  }
+  reset_stages(&c, 0);
+  return rc;
+}
+

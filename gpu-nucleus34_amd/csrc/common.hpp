@@ -222,10 +222,33 @@ class Allocator {
   uint64_t cached = 0;  // bytes held in the reuse cache
   bool caching = true;
   ~Allocator() { trim(); }
+  // Arena mode: every buffer is carved from [base, base + size) of device
+  // memory the caller owns, and the device allocator is never called. Blocks
+  // given back on top of the arena are reclaimed (buffers are released in
+  // reverse order of their requests, mostly); the rest when the arena's user
+  // is done with it.
+  void use_arena(void* base, size_t size) {
+    arena_ = static_cast<char*>(base);
+    arena_size_ = size;
+    arena_top_ = 0;
+  }
   DevBuf get(size_t bytes, const char* what) {
     DevBuf b;
     if (!bytes) return b;
     bytes = (bytes + 255) & ~static_cast<size_t>(255);
+    if (arena_) {
+      if (bytes > arena_size_ - arena_top_)
+        throw OutOfMemory(std::string("scratch array exhausted allocating ") + what + " (" + std::to_string(bytes) +
+                          " bytes; " + std::to_string(arena_top_) + " of " + std::to_string(arena_size_) +
+                          " in use)");
+      b.p_ = arena_ + arena_top_;
+      b.bytes_ = bytes;
+      b.owner_ = this;
+      arena_top_ += bytes;
+      used += bytes;
+      if (used > peak) peak = used;
+      return b;
+    }
     // reuse a cached block of at most twice the request
     auto it = free_.lower_bound(bytes);
     if (it != free_.end() && it->first <= 2 * bytes) {
@@ -267,6 +290,10 @@ class Allocator {
   // user on the single stream of the owning context.
   void give_back(void* p, size_t bytes) {
     used -= bytes;
+    if (arena_) {
+      if (static_cast<char*>(p) + bytes == arena_ + arena_top_) arena_top_ -= bytes;
+      return;
+    }
     if (caching) {
       free_.emplace(bytes, p);
       cached += bytes;
@@ -282,6 +309,8 @@ class Allocator {
 
  private:
   std::multimap<size_t, void*> free_;
+  char* arena_ = nullptr;
+  size_t arena_size_ = 0, arena_top_ = 0;
 };
 
 inline void DevBuf::release() {

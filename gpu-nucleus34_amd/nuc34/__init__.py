@@ -131,6 +131,8 @@ def _load() -> C.CDLL:
         "nuc34_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "nuc34_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)]),
         "nuc34_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
+        "nuc34_decompose_noalloc": (C.c_int, [C.c_int, u32p, u32p, C.c_uint32, C.c_uint64, C.c_int, C.c_uint64,
+                                              u32p, i32p, C.POINTER(Stats)]),
         "nuc34_serve_create": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(vp)]),
         "nuc34_serve_destroy": (None, [vp]),
         "nuc34_serve_now": (C.c_double, [vp]),
@@ -427,6 +429,31 @@ def decompose_graph(g: HostGraph, variant="auto", parts: int = 1, device: int = 
         ctx.load_graph(g)
         st = ctx.decompose(variant, parts)
         return ctx.copy_triangles(), ctx.copy_k(), st
+
+
+def decompose_noalloc(g, variant="auto", tri_capacity: int | None = None, device: int = 0):
+    """nuc34_decompose_noalloc: one undirected graph decomposed with every device
+    buffer in the library's per-device scratch array (no device allocation).
+    Returns (triangles (T,3), k (T,), stats); tri_capacity defaults to the
+    graph's wedge count, an upper bound on T."""
+    O, E = (g.O, g.E) if isinstance(g, HostGraph) else g
+    O = np.ascontiguousarray(O, dtype=np.uint32)
+    E = np.ascontiguousarray(E, dtype=np.uint32)
+    if tri_capacity is None:
+        d = np.diff(O.astype(np.int64))
+        tri_capacity = int((d * (d - 1) // 2).sum()) // 3 + 1
+    abc = np.empty((max(tri_capacity, 1), 3), dtype=np.uint32)
+    k = np.empty(max(tri_capacity, 1), dtype=np.int32)
+    st = Stats()
+    v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    rc = _lib.nuc34_decompose_noalloc(device, _u32(O), _u32(E if len(E) else O), len(O) - 1, len(E), v,
+                                      tri_capacity, _u32(abc), _i32(k), C.byref(st))
+    if rc != OK:
+        err = Nuc34Error(rc, (_lib.nuc34_last_error(None) or b"").decode(errors="replace"))
+        err.stats = st.as_dict()
+        raise err
+    T = st.triangles
+    return abc[:T], k[:T], st.as_dict()
 
 
 class Server:

@@ -117,7 +117,8 @@ int nuc34_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, dou
 int nuc34_ctx_create(int device, uint64_t hbm_budget, nuc34_ctx** out);
 void nuc34_ctx_destroy(nuc34_ctx* ctx);
 /* The message of the context's last failure; with ctx NULL, that of this
- * thread's last failed nuc34_ctx_create (for example no device). */
+ * thread's last failed nuc34_ctx_create (for example no device) or
+ * nuc34_decompose_noalloc. */
 const char* nuc34_last_error(const nuc34_ctx* ctx);
 
 /* Uploads a CSR. oriented == 0: undirected (both directions, lists sorted by
@@ -222,6 +223,31 @@ typedef struct nuc34_device_rates {
 } nuc34_device_rates;
 int nuc34_measure_device(int device, nuc34_device_rates* out);
 int nuc34_device_info(int device, char* buf, size_t len);
+
+/* ---- decomposition without device allocation ----------------------------- */
+
+/* Bytes of the per-device scratch array behind nuc34_decompose_noalloc. */
+#define NUC34_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it): the work of nuc34_load_csr (oriented = 0), nuc34_decompose and
+ * the copies of the triangles and k, for one undirected CSR, with every
+ * device buffer carved from a file-scope device array of
+ * NUC34_NOALLOC_SCRATCH_BYTES that each device the library runs on holds until
+ * the process ends. No hipMalloc, hipFree, stream, event, graph or pinned host
+ * buffer is created; work goes to the null stream and the call returns when
+ * it is done. abc receives 3 * T and k receives T entries, (a,b) order, when
+ * T <= tri_capacity; otherwise NUC34_E_INVALID, with stats->triangles = T.
+ * variant: NUC34_DIRECT, NUC34_ONDEMAND or NUC34_AUTO (on demand). Graphs whose
+ * input alone passes a quarter of the array are rejected with
+ * NUC34_E_INVALID; a graph whose intermediate state does not fit gets
+ * NUC34_E_OOM. Calls on one device share its array and are serialized by a
+ * per-device lock in this process (the array keeps the last call's contents;
+ * every call clears what it needs). Errors: nuc34_last_error(NULL). stats
+ * carries counts, rounds and the peak scratch bytes, no times. */
+int nuc34_decompose_noalloc(int device, const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, int variant,
+                            uint64_t tri_capacity, uint32_t* abc, int32_t* k, nuc34_stats* stats);
 
 /* ---- serving loop -------------------------------------------------------- */
 /*
