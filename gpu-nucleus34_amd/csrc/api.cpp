@@ -52,8 +52,7 @@ struct nuc34_ctx {
   DevBuf rowA, rowC, Aout, Ain, Cm;  // bit rows
   uint64_t WA = 0, WC = 0;           // words of the Aout/Ain rows and of the Cm rows
   DevBuf offA, triA, triBC;
-  DevBuf cIdx;                       // triangle id per c-slot
-  DevBuf ainpos;                     // per triangle: in-list position of its a-edge
+  DevBuf cSlot;                      // per c-slot: triangle id, in-list position of its a-edge
   // support
   bool has_sup = false;
   uint64_t Q = 0;
@@ -316,7 +315,7 @@ void reset_stages(nuc34_ctx* c, int from) {
   }
   if (from <= 1) {
     for (DevBuf* b : {&c->IO, &c->IE, &c->IEe, &c->inpos, &c->rowA, &c->rowC, &c->Aout, &c->Ain, &c->Cm, &c->offA,
-                      &c->triA, &c->triBC, &c->cIdx, &c->ainpos})
+                      &c->triA, &c->triBC, &c->cSlot})
       b->release();
     c->has_tris = false;
     c->T = 0;
@@ -360,8 +359,8 @@ BitRows bitrows(nuc34_ctx* c) {
 
 K4Src k4src(nuc34_ctx* c) {
   return K4Src{c->O.as<uint32_t>(),    c->E.as<uint32_t>(),     c->src.as<uint32_t>(), c->inpos.as<uint32_t>(),
-               bitrows(c),             c->triA.as<uint32_t>(),  c->triBC.as<uint64_t>(), c->cIdx.as<uint32_t>(),
-               c->ainpos.as<uint32_t>(), c->T,                  nullptr,                 nullptr,
+               bitrows(c),             c->triA.as<uint32_t>(),  c->triBC.as<uint64_t>(), c->cSlot.as<uint2>(),
+               c->T,                  nullptr,                 nullptr,
                nullptr};
 }
 
@@ -429,13 +428,12 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     }
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
-    c->cIdx = c->alloc.get(c->T * 4 + 4, "triangles by c-slot");
-    c->ainpos = c->alloc.get(c->T * 4 + 4, "a-edge in-list positions");
+    c->cSlot = c->alloc.get(c->T * 8 + 8, "triangles by c-slot");
     ktime(c, K_TRI_EMIT, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
                         c->inpos.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                        c->cIdx.as<uint32_t>(), c->ainpos.as<uint32_t>(), tri_tile(c), c->s);
+                        c->cSlot.as<uint2>(), tri_tile(c), c->s);
     });
   });
   c->st.ms_triangles = ms_enum + c->st.ms_tri_order;

@@ -13,7 +13,7 @@
 // word of edge e is offA[e] plus the set bits of e's row before that word, so
 // the id of the triangle at bit j of the row is rank + popc(bits below j) —
 // one 8-byte read; likewise a Cm word's rank is the c-slot offset cOff[e] plus
-// the bits before it, and cIdx[c-slot] holds the triangle id. With these:
+// the bits before it, and cSlot[c-slot] holds the triangle id. With these:
 //   * triangle index of (u, N+(u)[j], w) = row_rank(Ain row of (u,w), j)
 //   * K4s of t=(u,v,w) above u: bits of (Aout|Ain)[a] & (Aout|Ain)[b]
 //     K4s of t below u:          bits of Cm[a] & Cm[b]

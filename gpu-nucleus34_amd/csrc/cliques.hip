@@ -196,11 +196,12 @@ __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, ui
       q[2] = row_rank(s.rows.Ain + s.rows.rowA[exw], evx - s.O[x]);
     }
   } else {
-    q[0] = s.cIdx[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
-    q[1] = s.cIdx[rank_in_word(k.Ia, j)];  // T{x,u,w}: c-edge a
+    const uint2 xuv = s.cSlot[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
+    q[0] = xuv.x;
+    q[1] = s.cSlot[rank_in_word(k.Ia, j)].x;  // T{x,u,w}: c-edge a
     // T{x,v,w}: c-edge c, at x's position in N-(v), the in-list position of
     // (x,v), the a-edge of T{x,u,v}
-    q[2] = s.cIdx[row_rank(s.rows.Cm + r.cc, s.ainpos[q[0]])];
+    q[2] = s.cSlot[row_rank(s.rows.Cm + r.cc, xuv.y)].x;
   }
 }
 

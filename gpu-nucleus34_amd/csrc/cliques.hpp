@@ -16,8 +16,10 @@ struct K4Src {
   BitRows rows;
   const uint32_t* triA;   // a-edge per triangle
   const uint64_t* triBC;  // (b << 32) | c per triangle
-  const uint32_t* cIdx;   // triangle id per c-slot
-  const uint32_t* ainpos; // per triangle: inpos of its a-edge
+  // per c-slot: the triangle's id and the in-list position of its a-edge (a
+  // down K4's third partner is one rank lookup from the second, which this
+  // pair gives with its id in one load)
+  const uint2* cSlot;
   uint64_t T;
   // Live rows for the on-demand peel (null elsewhere), word for word beside
   // (Aout|Ain) and Cm: `live` holds the bits of the triangles not yet retired,

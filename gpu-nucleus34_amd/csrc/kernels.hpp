@@ -22,14 +22,14 @@ void launch_batch_union(const uint32_t* O, const uint32_t* E, const uint64_t* vf
 
 // (b) triangle enumeration over oriented edges [e_begin, e_end): the bits pass
 // fills the incidence rows, the emit pass (after the row ranks) writes the
-// triangles at their canonical indices, their ids at their c-slots and the
-// in-list position of their a-edge (ainpos). `tile` = oriented edges per wave
+// triangles at their canonical indices and, at their c-slots, their ids with
+// the in-list position of their a-edge (cSlot). `tile` = oriented edges per wave
 // tile: 16, 32 or 64 (std::logic_error otherwise).
 void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
                      const BitRows& rows, const uint32_t* inpos, uint32_t tile, hipStream_t s);
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
-                     uint32_t* ainpos, uint32_t tile, hipStream_t s);
+                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint2* cSlot,
+                     uint32_t tile, hipStream_t s);
 
 // abc[3i..3i+2] = (a, b, c) of triangles [0, count) of the given arrays.
 void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s);

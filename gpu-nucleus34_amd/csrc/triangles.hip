@@ -18,8 +18,9 @@
 //              a-role row of a (offA[a] plus the a-role bits below it), i.e.
 //              exactly its place in the reference's (a, b)-sorted triangle list
 //              (tri_t::operator<, utils.cuh:93-102), with no sort and no
-//              atomics; it is also stored at its c-slot (cIdx), the rank of the
-//              lowest vertex's in-list position in the c-role row of c.
+//              atomics; its id and its a-edge's in-list position are also
+//              stored at its c-slot (cSlot), the rank of the lowest vertex's
+//              in-list position in the c-role row of c.
 #include "kernels.hpp"
 
 namespace nuc34 {
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
                                                      uint64_t e_end, BitRows rows,
                                                      const uint32_t* __restrict__ inpos,
                                                      uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                                     uint32_t* __restrict__ cIdx, uint32_t* __restrict__ ainpos,
+                                                     uint2* __restrict__ cSlot,
                                                      uint32_t G) {
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
   __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
@@ -125,8 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
             const uint32_t p = row_rank(rows.Ain + rows.rowA[a], jv);
             triA[p] = a;
             triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
-            ainpos[p] = inpos[a];
-            cIdx[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = p;
+            cSlot[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = uint2{p, inpos[a]};
           }
         }
       }
@@ -163,17 +163,17 @@ void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, 
   if (e_end <= e_begin) return;
   const uint32_t G = tile_edges(tile);
   hipLaunchKernelGGL(k_tri_enum<false>, dim3(tri_grid(e_end - e_begin, G)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, inpos, nullptr, nullptr, nullptr, nullptr, G);
+                     e_end, rows, inpos, nullptr, nullptr, nullptr, G);
   NUC34_LAUNCH_CHECK();
 }
 
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint32_t* cIdx,
-                     uint32_t* ainpos, uint32_t tile, hipStream_t s) {
+                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint2* cSlot,
+                     uint32_t tile, hipStream_t s) {
   if (e_end <= e_begin) return;
   const uint32_t G = tile_edges(tile);
   hipLaunchKernelGGL(k_tri_enum<true>, dim3(tri_grid(e_end - e_begin, G)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, inpos, triA, triBC, cIdx, ainpos, G);
+                     e_end, rows, inpos, triA, triBC, cSlot, G);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -438,9 +438,11 @@ Result ondemand(const Graph& g, size_t parts) {
   // ORACLE34_PROGRESS set: one stderr line per phase and every 256 peel rounds
   // (long runs at full size stay visibly alive)
   const bool progress = std::getenv("ORACLE34_PROGRESS") != nullptr;
+  const auto t_start = clk::now();
   auto note = [&](const char* what, uint64_t a, uint64_t b) {
-    if (progress) std::fprintf(stderr, "[oracle34 ondemand] %s %llu %llu\n", what, static_cast<unsigned long long>(a),
-                               static_cast<unsigned long long>(b));
+    if (progress)
+      std::fprintf(stderr, "[oracle34 ondemand] %.1f s: %s %llu %llu\n", secs(t_start, clk::now()), what,
+                   static_cast<unsigned long long>(a), static_cast<unsigned long long>(b));
   };
   Result r;
   const size_t P = parts ? parts : static_cast<size_t>(omp_get_num_procs());
