@@ -713,9 +713,12 @@ __device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
   }
 }
 
+// State words per thread and pass of a level scan.
+constexpr int kScanWords = 4;
+
 template <bool UNLINK>
 __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
-  __shared__ uint32_t wd[kWPB], base;
+  __shared__ uint32_t wd[kScanWords][kWPB], base;
   if (in->done) return;
   const uint32_t F = in->ncur + in->nspec;
   const bool collect_round = F == 0;
@@ -741,41 +744,55 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
     const uint64_t W = (p.T + 15) / 16;
     const unsigned wv = threadIdx.x / kWave;
     uint32_t mn = kNoMin;
-    for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kBlock; b0 < W; b0 += static_cast<uint64_t>(gridDim.x) * kBlock) {
-      const uint64_t i = b0 + threadIdx.x;
-      uint32_t die = 0;
-      if (i < W) {
-        const uint32_t al = alive_fields(p.st[i]);
-        if (al) {
+    // kScanWords state words per thread and pass (words b0 + k * kBlock +
+    // thread), their loads issued together; one block prefix and one
+    // collection-count atomic per pass, so the collected triangles stay in
+    // index order within the block's kScanWords * kBlock words
+    for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kBlock * kScanWords; b0 < W;
+         b0 += static_cast<uint64_t>(gridDim.x) * kBlock * kScanWords) {
+      uint32_t al[kScanWords], die[kScanWords], nd[kScanWords], id[kScanWords];
+#pragma unroll
+      for (int k = 0; k < kScanWords; ++k) {
+        const uint64_t i = b0 + static_cast<uint64_t>(k) * kBlock + threadIdx.x;
+        al[k] = i < W ? alive_fields(p.st[i]) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kScanWords; ++k) {
+        die[k] = 0;
+        if (al[k]) {
           int32_t d[16];
-          load_word_dd(p.dd, i, d);
+          load_word_dd(p.dd, b0 + static_cast<uint64_t>(k) * kBlock + threadIdx.x, d);
 #pragma unroll
           for (int f = 0; f < 16; ++f) {
-            if (!((al >> (2 * f)) & 1u)) continue;
+            if (!((al[k] >> (2 * f)) & 1u)) continue;
             mn = min(mn, static_cast<uint32_t>(d[f] < 0 ? 0 : d[f]));
-            if (d[f] <= take) die |= 1u << f;
+            if (d[f] <= take) die[k] |= 1u << f;
           }
         }
+        nd[k] = static_cast<uint32_t>(__popc(die[k]));
+        id[k] = wave_inclusive_sum(nd[k]);
+        if (lane_id() == kWave - 1) wd[k][wv] = id[k];
       }
-      const uint32_t nd = static_cast<uint32_t>(__popc(die));
-      const uint32_t id = wave_inclusive_sum(nd);
-      if (lane_id() == kWave - 1) wd[wv] = id;
       __syncthreads();
       if (threadIdx.x == 0) {
         uint32_t td = 0;
-        for (int q = 0; q < kWPB; ++q) {
-          const uint32_t x = wd[q];
-          wd[q] = td;
-          td += x;
-        }
+        for (int k = 0; k < kScanWords; ++k)
+          for (int q = 0; q < kWPB; ++q) {
+            const uint32_t x = wd[k][q];
+            wd[k][q] = td;
+            td += x;
+          }
         base = td ? atomic_add_u32(&out->nspec, td) : 0;
       }
       __syncthreads();
-      if (die) {
-        uint32_t od = base + wd[wv] + id - nd;
+#pragma unroll
+      for (int k = 0; k < kScanWords; ++k) {
+        if (!die[k]) continue;
+        const uint64_t i = b0 + static_cast<uint64_t>(k) * kBlock + threadIdx.x;
+        uint32_t od = base + wd[k][wv] + id[k] - nd[k];
         uint32_t set = 0;
         for (int f = 0; f < 16; ++f) {
-          if (!((die >> f) & 1u)) continue;
+          if (!((die[k] >> f) & 1u)) continue;
           const uint32_t t = static_cast<uint32_t>(16 * i + f);
           p.kv[t] = take;
           nxt[od++] = t;
@@ -783,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
         }
         atomic_or_u32(&p.st[i], set);  // 00 -> 01
         if (UNLINK) {
-          for (uint32_t d = die; d; d &= d - 1u) mark_dying(s, static_cast<uint32_t>(16 * i + __builtin_ctz(d)));
+          for (uint32_t d = die[k]; d; d &= d - 1u) mark_dying(s, static_cast<uint32_t>(16 * i + __builtin_ctz(d)));
         }
       }
       __syncthreads();
