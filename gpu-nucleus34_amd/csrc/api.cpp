@@ -1289,16 +1289,19 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
   if (v != NUC34_DIRECT && v != NUC34_ONDEMAND) return fail(c, NUC34_E_INVALID, "unknown variant");
   nuc34_tuning cur;
   nuc34_get_tuning(c, &cur);
+  // a candidate's time: the median of three whole decompositions
   auto time_of = [&](const nuc34_tuning& t, double* ms) -> int {
     c->tune = t;
-    *ms = 1e300;
-    for (int r = 0; r < 2; ++r) {
+    double runs[3];
+    for (double& r_ms : runs) {
       int rc = nuc34_reset(c);
       nuc34_stats st{};
       if (rc == NUC34_OK) rc = nuc34_decompose(c, v, 1, &st);
       if (rc != NUC34_OK) return rc;
-      *ms = std::min(*ms, st.ms_total);
+      r_ms = st.ms_total;
     }
+    std::sort(runs, runs + 3);
+    *ms = runs[1];
     return NUC34_OK;
   };
   double cur_ms = 0;
@@ -1317,20 +1320,28 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
       {&nuc34_tuning::tri_tile_edges, {16, 32, 64}, false, false},
       {&nuc34_tuning::peel_finish_grid, {256, 512, 1024}, false, false},
   };
-  for (const Knob& k : knobs) {
-    if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND)) continue;
-    for (uint32_t val : k.values) {
-      if (cur.*k.field == val) continue;
-      nuc34_tuning t = cur;
-      t.*k.field = val;
-      double ms = 0;
-      rc = time_of(t, &ms);
-      if (rc != NUC34_OK) return rc;
-      if (ms < cur_ms) {
-        cur = t;
-        cur_ms = ms;
+  // coordinate descent, repeated until a pass changes nothing (at most three
+  // passes); a candidate replaces the current shape only when it is faster by
+  // more than 1%, so near-equal shapes do not flip the choice between runs
+  for (int pass = 0; pass < 3; ++pass) {
+    bool changed = false;
+    for (const Knob& k : knobs) {
+      if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND)) continue;
+      for (uint32_t val : k.values) {
+        if (cur.*k.field == val) continue;
+        nuc34_tuning t = cur;
+        t.*k.field = val;
+        double ms = 0;
+        rc = time_of(t, &ms);
+        if (rc != NUC34_OK) return rc;
+        if (ms < 0.99 * cur_ms) {
+          cur = t;
+          cur_ms = ms;
+          changed = true;
+        }
       }
     }
+    if (!changed) break;
   }
   c->tune = cur;
   if (best) *best = cur;

@@ -200,9 +200,10 @@ typedef struct nuc34_tuning {
 int nuc34_set_tuning(nuc34_ctx* ctx, const nuc34_tuning* tuning);
 int nuc34_get_tuning(const nuc34_ctx* ctx, nuc34_tuning* tuning);
 /* Times candidate launch shapes on the loaded graph, one parameter at a time
- * from the current setting (each candidate: two whole decompositions in
- * `variant`, the faster counts), keeps the fastest in the context and returns
- * it with its decomposition time in milliseconds. The reference launches with
+ * from the current setting, in passes until a pass changes nothing (at most
+ * three; each candidate: the median of three whole decompositions in
+ * `variant`, adopted when more than 1% faster), keeps the fastest in the
+ * context and returns it with its decomposition time in milliseconds. The reference launches with
  * fixed shapes (MAX_THRD_BLK, nucleus/utils.cuh:29); no counterpart there. */
 int nuc34_autotune(nuc34_ctx* ctx, int variant, nuc34_tuning* best, double* best_ms);
 

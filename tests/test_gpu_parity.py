@@ -317,6 +317,45 @@ def test_full_batch_512_graphs_vs_oracle():
             assert np.array_equal(k[sl], r["k"]), (variant, j)
 
 
+def test_fourclique_copy_fits_beside_the_direct_peel():
+    """Copying the first K4s after a direct decomposition needs O(T) more
+    device memory, not O(Q): it succeeds with a budget just above the
+    decomposition's peak (a dense graph, so 8 bytes per K4 would not fit), and
+    the K4s equal those of an unconstrained context."""
+    n = 70
+    pairs = np.array([(i, j) for i in range(n) for j in range(i + 1, n) if (i * 7 + j * 3) % 11], dtype=np.uint32)
+    g = nuc34.graph_from_pairs(pairs[:, 0], pairs[:, 1])
+    with nuc34.Context(0) as ctx:
+        ctx.load_graph(g)
+        st = ctx.decompose("direct")
+        want = ctx.copy_fourcliques(0, 5)
+        mid = ctx.copy_fourcliques(st["four_cliques"] // 2, 7)
+    T, Q = st["triangles"], st["four_cliques"]
+    assert 8 * Q > 64 * (T + 1) + (1 << 20)
+    budget = st["peak_bytes"] + 24 * (T + 1) + (256 << 10)
+    with nuc34.Context(0, budget) as ctx:
+        ctx.load_graph(g)
+        ctx.decompose("direct")
+        assert np.array_equal(ctx.copy_fourcliques(0, 5), want)
+        assert np.array_equal(ctx.copy_fourcliques(Q // 2, 7), mid)
+
+
+def test_batch_graph_index_bounds():
+    """nuc34_batch_graph rejects indices past the last graph, 2^32 - 1 included
+    (the index is compared in 64 bits)."""
+    import ctypes as C
+    gs = [nuc34.gen_batch_graph(3, i) for i in range(2)]
+    with nuc34.Context() as ctx:
+        ctx.load_batch(gs)
+        ctx.decompose("auto")
+        vf = C.c_uint64()
+        for g in (2, 0xFFFFFFFF):
+            rc = nuc34._lib.nuc34_batch_graph(ctx._h, g, C.byref(vf), None, None, None)
+            assert rc == nuc34.E_INVALID, g
+        assert nuc34._lib.nuc34_batch_graph(ctx._h, 1, C.byref(vf), None, None, None) == nuc34.OK
+        assert vf.value == gs[0].n
+
+
 def test_batch_concatenated_and_pointer_loads_agree():
     """nuc34_load_batch (graphs back to back) and nuc34_load_batch_ptrs (one
     pointer pair per graph, an edgeless graph among them) load the same union."""
