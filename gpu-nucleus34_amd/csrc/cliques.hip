@@ -847,10 +847,34 @@ __device__ __forceinline__ void prepare_out_slot(PeelCtl* out, bool done) {
   }
 }
 
-// Direct debit: one wave per dying triangle, one K4 list entry per lane.
+// Per-wave LDS of the direct debit: the list extents of the tile's triangles.
+struct ListTile {
+  uint64_t pre[kWave + 1];  // first entry of each triangle in the tile's flattened entries
+  uint64_t first[kWave];    // koff of each triangle
+  uint32_t tri[kWave];
+};
+
+// Largest index i < 64 with pre[i] <= key (pre non-decreasing, pre[0] = 0):
+// the triangle holding flattened entry key (empty triangles share their
+// successor's start and are skipped).
+__device__ __forceinline__ uint32_t find_slot_u64(const uint64_t* pre, uint64_t key) {
+  uint32_t lo = 0, hi = kWave;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Direct debit: a wave takes tile_width() dying triangles and flattens their
+// K4 list entries over its lanes, two entries per lane and pass, so a short
+// list does not leave most of a wave idle. Both entries' loads, then both
+// entries' state reads, are issued before either is waited on.
 __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
                                                              const uint64_t* __restrict__ koff,
-                                                             const uint32_t* __restrict__ lists) {
+                                                             const uint32_t* __restrict__ lists, uint32_t tile_cap) {
+  __shared__ ListTile lds[kWPB];
   const bool done = in->done;
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
@@ -859,26 +883,50 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   const uint32_t n = done ? 0u : in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
-  const uint32_t nw = gridDim.x * kWPB;
-  for (uint32_t i = blockIdx.x * kWPB + w; i < n; i += nw) {
-    const uint32_t t = cur[i];
-    const uint64_t b = koff[t], e = koff[t + 1];
-    // two entries per lane and pass: both entries' loads, then both entries'
-    // state reads, are issued before either is waited on
-    for (uint64_t j = b + lane; j < e; j += 2 * kWave) {
-      const bool two = j + kWave < e;
+  ListTile& L = lds[w];
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
+  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
+    const uint64_t i = tile * G + lane;
+    uint32_t t = 0;
+    uint64_t b = 0, len = 0;
+    if (lane < G && i < n) {
+      t = cur[i];
+      b = koff[t];
+      len = koff[t + 1] - b;
+    }
+    const uint64_t inc = wave_inclusive_sum_u64(len);
+    const uint64_t total = __shfl(inc, kWave - 1);
+    L.pre[lane] = inc - len;
+    if (lane == kWave - 1) L.pre[kWave] = inc;
+    L.first[lane] = b;
+    L.tri[lane] = t;
+    wave_sync();
+    for (uint64_t e = lane; e < total; e += 2 * kWave) {
+      const bool two = e + kWave < total;
+      const uint32_t sa_ = find_slot_u64(L.pre, e);
+      const uint32_t ta = L.tri[sa_];
+      const uint64_t ja = L.first[sa_] + (e - L.pre[sa_]);
+      uint32_t tb = 0;
+      uint64_t jb = 0;
+      if (two) {
+        const uint32_t sb_ = find_slot_u64(L.pre, e + kWave);
+        tb = L.tri[sb_];
+        jb = L.first[sb_] + (e + kWave - L.pre[sb_]);
+      }
       uint32_t qa[3], qb[3] = {0u, 0u, 0u}, sa[3], sb[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) qa[k] = __builtin_nontemporal_load(lists + 3 * j + k);
+      for (int k = 0; k < 3; ++k) qa[k] = __builtin_nontemporal_load(lists + 3 * ja + k);
       if (two) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) qb[k] = __builtin_nontemporal_load(lists + 3 * (j + kWave) + k);
+        for (int k = 0; k < 3; ++k) qb[k] = __builtin_nontemporal_load(lists + 3 * jb + k);
       }
       read_states(p, qa, sa);
       if (two) read_states(p, qb, sb);
-      debit_k4(t, qa, sa, p, level, nxt, ncount);
-      if (two) debit_k4(t, qb, sb, p, level, nxt, ncount);
+      debit_k4(ta, qa, sa, p, level, nxt, ncount);
+      if (two) debit_k4(tb, qb, sb, p, level, nxt, ncount);
     }
+    wave_sync();
   }
   prepare_out_slot(out, done);
 }
@@ -1000,7 +1048,8 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
                               const PeelShape& shape, hipStream_t s) {
   PeelCtl* in = p.ctl + (slot & 1);
   PeelCtl* out = p.ctl + ((slot + 1) & 1);
-  hipLaunchKernelGGL(k_peel_debit_lists, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, koff, lists);
+  hipLaunchKernelGGL(k_peel_debit_lists, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, koff, lists,
+                     static_cast<uint32_t>(shape.tile_cap ? shape.tile_cap : 1));
   hipLaunchKernelGGL(k_peel_finish<false>, dim3(shape.finish_grid), dim3(kBlock), 0, s, p, in, out, K4Src{});
   NUC34_LAUNCH_CHECK();
 }

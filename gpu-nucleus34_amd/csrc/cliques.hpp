@@ -111,8 +111,8 @@ struct PeelBufs {
 // round's control says done, so extra rounds are no-ops. The direct round
 // reads the K4 lists; the on-demand round enumerates each dying triangle's K4s
 // from the bit rows.
-// Launch shape of the rounds: workgroups of the debit kernel and, on demand,
-// the largest number of dying triangles one wave takes at a time.
+// Launch shape of the rounds: workgroups of the debit kernel and the largest
+// number of dying triangles one wave takes at a time.
 struct PeelShape {
   unsigned debit_grid = 2048;
   unsigned tile_cap = 64;

@@ -87,21 +87,26 @@ __global__ void k_fill(T* __restrict__ a, uint64_t n, T v) {
     a[i] = v;
 }
 
-// Undirected CSR check, thread per vertex: offsets ordered and within m, each
+// Undirected CSR check, wave per vertex (coalesced over the list, so a hub
+// vertex does not serialize one thread): offsets ordered and within m, each
 // list strictly increasing with ids < n and no self-loop; *bad = lowest
 // offending vertex. A list is read only after its offsets passed, so every
 // read stays inside O[0..n] and E[0..m).
-__global__ void k_check_csr(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E, uint32_t n, uint64_t m,
-                            uint32_t* __restrict__ bad) {
-  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
-       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+__global__ __launch_bounds__(kBlock) void k_check_csr(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
+                                                      uint32_t n, uint64_t m, uint32_t* __restrict__ bad) {
+  const unsigned lane = lane_id();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / kWave);
+  for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; u < n;
+       u += waves) {
     const uint32_t o0 = O[u], o1 = O[u + 1];
     bool ok = o0 <= o1 && o1 <= m;
-    for (uint32_t i = o0; ok && i < o1; ++i) {
-      const uint32_t x = E[i];
-      ok = x < n && x != u && (i == o0 || E[i - 1] < x);
+    if (ok) {
+      for (uint32_t i = o0 + lane; i < o1; i += kWave) {
+        const uint32_t x = E[i];
+        ok = ok && x < n && x != u && (i == o0 || E[i - 1] < x);
+      }
     }
-    if (!ok) atomic_min_u32(bad, static_cast<uint32_t>(u));
+    if (__ballot(!ok) && lane == 0) atomic_min_u32(bad, static_cast<uint32_t>(u));
   }
 }
 
@@ -159,7 +164,7 @@ __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint64_t* _
 
 void launch_check_csr(const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t m, uint32_t* bad, hipStream_t s) {
   if (!n) return;
-  hipLaunchKernelGGL(k_check_csr, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, O, E, n, m, bad);
+  hipLaunchKernelGGL(k_check_csr, dim3(grid_for(n, kBlock / kWave, 1u << 16)), dim3(kBlock), 0, s, O, E, n, m, bad);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -190,7 +190,7 @@ int nuc34_reset(nuc34_ctx* ctx);
 /* Launch shapes a context uses; 0 in any field selects its default. */
 typedef struct nuc34_tuning {
   uint32_t peel_debit_grid;        /* workgroups of a peel round's debit kernel (2048) */
-  uint32_t peel_tile_cap;          /* on-demand: most dying triangles one wave takes at a time, 1..64 (64) */
+  uint32_t peel_tile_cap;          /* most dying triangles one wave of the debit kernel takes at a time, 1..64 (64) */
   uint32_t peel_rounds_per_launch; /* peel rounds per captured graph launch, even (32) */
   uint32_t k4_list_grid_cap;       /* most workgroups of the direct K4 list build (32768) */
   uint32_t tri_tile_edges;         /* oriented edges per wave tile of the triangle intersection: 16, 32, 64 (64) */
