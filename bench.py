@@ -34,7 +34,12 @@ Prints one JSON line with the driver's contract, plus:
   batched       512 graphs of the seeded mixed batch decomposed as one disjoint
                 union versus one decomposition per graph (graphs/s, bit-exact);
   device_rates  measured copy GB/s and random 4-byte atomic / load rates;
-  config.kernel_gbs      algorithmic GB/s of every timed kernel group;
+  config.kernel_gbs      algorithmic GB/s of every timed kernel group, by the
+                survey's per-unit formulas, which price the reference's merge over
+                incidence lists of length l(e): where this build reads bit rows
+                instead (the support count) the model rate can pass the HBM peak;
+  config.kernel_physical_gbs  measured HBM GB/s of the same groups (PMC fetch +
+                write bytes of the profiled step over the timed kernel time);
   config.end_to_end_ms   parse, load (host to device, checks, orientation),
                 decomposition and the copy of k and the triangles back.
 """
@@ -200,12 +205,14 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
             print(f"traffic: {counter} database has no calibration copy (exit {rc}, {len(rows)} rows)", file=sys.stderr)
             return None
         scale = calib_mib * 1024.0 / max(copies)  # counter KiB -> true KiB
-        names = GROUP_KERNELS.get(group, ())
-        kib = sum(v for n, v in rows if any(k in n for k in names))
-        res[counter] = (kib * scale * 1024.0 / launches, scale)
+        per_group = {g: sum(v for n, v in rows if any(k in n for k in names)) * scale * 1024.0
+                     for g, names in GROUP_KERNELS.items()}  # bytes in the one profiled step
+        res[counter] = (per_group[group] / launches, scale, per_group)
     return {"bytes_per_launch": res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0],
             "fetch_bytes_per_launch": res["FETCH_SIZE"][0], "write_bytes_per_launch": res["WRITE_SIZE"][0],
             "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1],
+            "group_bytes_per_step": {g: res["FETCH_SIZE"][2][g] + res["WRITE_SIZE"][2][g]
+                                     for g in GROUP_KERNELS if res["FETCH_SIZE"][2][g] + res["WRITE_SIZE"][2][g] > 0},
             "calibration": f"{calib_mib} MiB device-to-device copy in the profiled process"}
 
 
@@ -558,6 +565,10 @@ def main():
     # achieved algorithmic GB/s of every timed kernel group (intersection, support, lists, peel)
     kernel_gbs = {k: round(algorithmic_bytes(k, gs, st) / (v["ms"] / args.steps / 1e3) / 1e9, 1)
                   for k, v in kern.items() if v["ms"] > 0 and algorithmic_bytes(k, gs, st) > 0}
+    # physical GB/s (PMC fetch + write bytes of the profiled step over the timed step's kernel time)
+    kernel_physical_gbs = ({k: round(b / (kern[k]["ms"] / args.steps / 1e3) / 1e9, 1)
+                            for k, b in traffic["group_bytes_per_step"].items() if kern.get(k, {}).get("ms", 0) > 0}
+                           if traffic else None)
     per_launch_ms = kern[dom]["ms"] / kern[dom]["launches"]
     bytes_per_launch = algorithmic_bytes(dom, gs, st) / launches_per_step
     achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
@@ -582,7 +593,8 @@ def main():
                                                            "ms_incidence", "ms_peel", "ms_total")},
                    "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 3) for k, v in kern.items()},
                    "kernel_launches_per_step": {k: v["launches"] / args.steps for k, v in kern.items()},
-                   "kernel_gbs": kernel_gbs, "end_to_end_ms": {k: round(v, 3) for k, v in end_to_end.items()}},
+                   "kernel_gbs": kernel_gbs, "kernel_physical_gbs": kernel_physical_gbs,
+                   "end_to_end_ms": {k: round(v, 3) for k, v in end_to_end.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
