@@ -516,11 +516,11 @@ void run_peel(nuc34_ctx* c, int variant) {
   K4Src v = k4src(c);
   DevBuf liveA, liveC, icut;
   if (variant != NUC34_DIRECT) {
-    liveA = c->alloc.get(c->WA * 8 + 8, "live a/b rows");
-    liveC = c->alloc.get(c->WC * 8 + 8, "live c rows");
+    liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
+    liveC = c->alloc.get(c->WC * sizeof(PeelWord) + sizeof(PeelWord), "peel c rows");
     icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
-    v.LA = liveA.as<LiveWord>();
-    v.LC = liveC.as<LiveWord>();
+    v.PA = liveA.as<PeelWord>();
+    v.PC = liveC.as<PeelWord>();
     v.icut = icut.as<uint32_t>();
   }
   // round parity must equal slot parity: groups of even size

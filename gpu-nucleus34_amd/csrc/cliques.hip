@@ -139,10 +139,10 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     const uint64_t rb = r.ru0 + static_cast<uint64_t>(r.jv) * r.W + wi;
     k.Aa = s.rows.Aout[ra];
     k.Ab = s.rows.Aout[rb];
-    k.Ia = s.rows.Ain[ra];
-    k.Ib = s.rows.Ain[rb];
     if (LIVE) {
-      const LiveWord la = s.LA[ra], lb = s.LA[rb];
+      const PeelWord la = s.PA[ra], lb = s.PA[rb];
+      k.Ia = RankWord{la.bits, la.rank};
+      k.Ib = RankWord{lb.bits, lb.rank};
       const uint32_t bw = bits_below(r.jw, wi * 32), bv = bits_below(r.jv, wi * 32);
       const uint32_t lower_b = (k.Ab & bw) | (~k.Ab & (r.jv < r.jw ? ~0u : 0u));
       const uint32_t lower_a = (k.Aa & bw) | (~k.Aa & bv);
@@ -150,6 +150,8 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
       k.Da = la.dying;
       k.Db = lb.dying;
     } else {
+      k.Ia = s.rows.Ain[ra];
+      k.Ib = s.rows.Ain[rb];
       k.bits = (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
       k.Da = k.Db = 0;
     }
@@ -158,14 +160,16 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     const uint64_t ca = r.ca + (wi - r.W);
     const uint64_t cb = ca + static_cast<uint64_t>(r.jv) * r.WC - static_cast<uint64_t>(r.jw) * r.WC;
     k.Aa = k.Ab = 0;
-    k.Ia = s.rows.Cm[ca];
-    k.Ib = s.rows.Cm[cb];
     if (LIVE) {
-      const LiveWord la = s.LC[ca], lb = s.LC[cb];
+      const PeelWord la = s.PC[ca], lb = s.PC[cb];
+      k.Ia = RankWord{la.bits, la.rank};
+      k.Ib = RankWord{lb.bits, lb.rank};
       k.bits = la.live & lb.live & ~((la.dying | lb.dying) & bits_below(r.cut, (wi - r.W) * 32));
       k.Da = la.dying;
       k.Db = lb.dying;
     } else {
+      k.Ia = s.rows.Cm[ca];
+      k.Ib = s.rows.Cm[cb];
       k.bits = k.Ia.bits & k.Ib.bits;
       k.Da = k.Db = 0;
     }
@@ -173,35 +177,61 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
   return k;
 }
 
-// The three partner triangles of the K4 at bit j of word wi.
+// Rank of bit position j of a peel row; *state gets that bit's live (bit 0)
+// and dying (bit 1) planes.
+__device__ __forceinline__ uint32_t peel_rank(const PeelWord* row, uint32_t j, uint32_t* state) {
+  const PeelWord w = row[j >> 5];
+  const uint32_t b = j & 31u;
+  *state = ((w.live >> b) & 1u) | (((w.dying >> b) & 1u) << 1);
+  return w.rank + static_cast<uint32_t>(__popc(w.bits & ((1u << b) - 1u)));
+}
+
+// Rank of bit position j of the a-role row at word offset `row`: from the
+// peel rows with LIVE (the on-demand peel), from Ain elsewhere.
+template <bool LIVE>
+__device__ __forceinline__ uint32_t a_rank(const K4Src& s, uint64_t row, uint32_t j) {
+  if (LIVE) {
+    const PeelWord& w = s.PA[row + (j >> 5)];
+    return rank_in_word(RankWord{w.bits, w.rank}, j & 31u);
+  }
+  return row_rank(s.rows.Ain + row, j);
+}
+
+// The three partner triangles of the K4 at bit j of word wi. With LIVE, *s2
+// gets q[2]'s live and dying planes (peel_rank) from the word its rank comes
+// from.
+template <bool LIVE>
 __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, uint32_t wi, const K4Word& k,
-                                            uint32_t j, uint32_t q[3]) {
+                                            uint32_t j, uint32_t q[3], uint32_t* s2) {
   if (wi < r.W) {
     const uint32_t jx = wi * 32 + j;
     const bool vx = (k.Ab >> j) & 1u;  // v ranks below x
     const bool wx = (k.Aa >> j) & 1u;  // w ranks below x
-    const RankWord* Ix = s.rows.Ain + r.ru0 + static_cast<uint64_t>(jx) * r.W;  // a-role row of (u, x)
-    q[0] = vx ? row_rank(Ix, r.jv) : rank_in_word(k.Ib, j);  // T{u,v,x}
-    q[1] = wx ? row_rank(Ix, r.jw) : rank_in_word(k.Ia, j);  // T{u,w,x}
+    const uint64_t ix = r.ru0 + static_cast<uint64_t>(jx) * r.W;  // a-role row of (u, x)
+    q[0] = vx ? a_rank<LIVE>(s, ix, r.jv) : rank_in_word(k.Ib, j);  // T{u,v,x}
+    q[1] = wx ? a_rank<LIVE>(s, ix, r.jw) : rank_in_word(k.Ia, j);  // T{u,w,x}
     const uint32_t evx = static_cast<uint32_t>(s.triBC[q[0]]);  // c-edge of T{u,v,x}: v-x
+    uint64_t row2;
+    uint32_t pos2;
     if (vx) {
-      const RankWord* Iv = s.rows.Ain + r.rv0;
-      q[2] = wx ? row_rank(Iv + static_cast<uint64_t>(evx - r.ov) * r.Wv, r.c - r.ov)  // v<w<x: a=(v,x), b=(v,w)
-                : row_rank(Iv + static_cast<uint64_t>(r.c - r.ov) * r.Wv, evx - r.ov);  // v<x<w: a=(v,w), b=(v,x)
+      row2 = r.rv0 + static_cast<uint64_t>(wx ? evx - r.ov : r.c - r.ov) * r.Wv;
+      pos2 = wx ? r.c - r.ov : evx - r.ov;  // v<w<x: a=(v,x), b=(v,w); v<x<w: a=(v,w), b=(v,x)
     } else {
       // x<v<w: a=(x,w), b=(x,v); x = N+(u)[jx] (consecutive for neighbouring
       // lanes), v's position in N+(x) from the edge id of (x,v)
       const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);
       const uint32_t x = s.E[r.ou + jx];
-      q[2] = row_rank(s.rows.Ain + s.rows.rowA[exw], evx - s.O[x]);
+      row2 = s.rows.rowA[exw];
+      pos2 = evx - s.O[x];
     }
+    q[2] = LIVE ? peel_rank(s.PA + row2, pos2, s2) : row_rank(s.rows.Ain + row2, pos2);
   } else {
     const uint2 xuv = s.cSlot[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
     q[0] = xuv.x;
     q[1] = s.cSlot[rank_in_word(k.Ia, j)].x;  // T{x,u,w}: c-edge a
     // T{x,v,w}: c-edge c, at x's position in N-(v), the in-list position of
     // (x,v), the a-edge of T{x,u,v}
-    q[2] = s.cSlot[row_rank(s.rows.Cm + r.cc, xuv.y)].x;
+    q[2] = s.cSlot[LIVE ? peel_rank(s.PC + r.cc, xuv.y, s2) : row_rank(s.rows.Cm + r.cc, xuv.y)].x;
   }
 }
 
@@ -264,7 +294,8 @@ struct TileLds {
 // emit(t, aux, pos, q, die) runs once per K4 of each triangle. With POS, pos is the K4's index in the
 // triangle's list (up words first, word and bit order); without, pos is 0. With
 // LIVE, bit k of die says that q[k] (k = 0, 1: the partners on t's own rows)
-// dies in the current round; 0 elsewhere.
+// dies in the current round, bit 2 that q[2] is not yet retired and bit 3
+// that q[2] dies in the current round; 0 elsewhere.
 template <bool POS, bool LIVE, typename Emit>
 __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, uint64_t aux, TileLds& L,
                                         Emit&& emit) {
@@ -325,10 +356,10 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
       if (e < nk) {
         const uint32_t r = e - esl;
         const uint32_t j = select_bit(kw.bits, r);
-        uint32_t q[3];
-        k4_partners(s, rt, wsl, kw, j, q);
+        uint32_t q[3], s2 = 0;
+        k4_partners<LIVE>(s, rt, wsl, kw, j, q, &s2);
         // q[0] sits on b's row, q[1] on a's, at bit j of this word
-        const uint32_t die = LIVE ? (((kw.Db >> j) & 1u) | (((kw.Da >> j) & 1u) << 1)) : 0u;
+        const uint32_t die = LIVE ? (((kw.Db >> j) & 1u) | (((kw.Da >> j) & 1u) << 1) | (s2 << 2)) : 0u;
         emit(tt, ta, POS ? psl + r : 0u, q, die);
       }
     }
@@ -630,20 +661,23 @@ __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const 
 }
 
 // On-demand variant: the K4 came from t's live rows, so q[0] and q[1] are not
-// gone, the dying planes say whether each dies in this round (`die`), and the
-// word-level check has already dropped the K4 if one of them owns it. Only
-// q[2]'s state is read. A row partner that is not dying is debited: it is
-// alive or was pushed earlier in this round, and a pushed triangle's k is
-// fixed, so its extra debit changes nothing (it cannot be pushed again).
+// gone, the dying planes say whether each dies in this round (`die` bits 0, 1),
+// and the word-level check has already dropped the K4 if one of them owns it.
+// q[2]'s planes came with its rank (`die` bits 2, 3): retired means the K4 is
+// gone, dying with a lower index means q[2] owns it. A partner that is not
+// dying is debited: it is alive or was pushed earlier in this round, and a
+// pushed triangle's k is fixed, so its extra debit changes nothing (it cannot
+// be pushed again). The planes change only between rounds, so no read here
+// races with a write.
 __device__ __forceinline__ void destroy_k4_live(uint32_t t, const uint32_t q[3], uint32_t die, const PeelBufs& p,
                                                 int32_t level, uint32_t* __restrict__ nxt,
                                                 uint32_t* __restrict__ ncount) {
-  const uint32_t s2 = tri_state(p.st, q[2]);
-  if (s2 == kGone || (s2 == kCur && q[2] < t)) return;
+  const bool live2 = (die >> 2) & 1u, dying2 = (die >> 3) & 1u;
+  if (!live2 || (dying2 && q[2] < t)) return;
   bool live[3];
   live[0] = !(die & 1u);
   live[1] = !(die & 2u);
-  live[2] = s2 == kAlive;
+  live[2] = !dying2;
   int32_t old[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -697,19 +731,19 @@ __device__ __forceinline__ TriSlots tri_slots(const K4Src& s, uint32_t t) {
 // t joins the current frontier: its bits in the dying planes.
 __device__ __forceinline__ void mark_dying(const K4Src& s, uint32_t t) {
   const TriSlots x = tri_slots(s, t);
-  atomic_or_u32(&s.LA[x.w[0]].dying, 1u << x.bit[0]);
-  atomic_or_u32(&s.LA[x.w[1]].dying, 1u << x.bit[1]);
-  atomic_or_u32(&s.LC[x.w[2]].dying, 1u << x.bit[2]);
+  atomic_or_u32(&s.PA[x.w[0]].dying, 1u << x.bit[0]);
+  atomic_or_u32(&s.PA[x.w[1]].dying, 1u << x.bit[1]);
+  atomic_or_u32(&s.PC[x.w[2]].dying, 1u << x.bit[2]);
 }
 
 // t is retired: its bits leave both planes (one 64-bit atomic per word).
 __device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
   const TriSlots x = tri_slots(s, t);
-  LiveWord* const words[3] = {s.LA + x.w[0], s.LA + x.w[1], s.LC + x.w[2]};
+  PeelWord* const words[3] = {s.PA + x.w[0], s.PA + x.w[1], s.PC + x.w[2]};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const uint64_t bit = 1ull << x.bit[k];
-    atomic_and_u64(reinterpret_cast<uint64_t*>(words[k]), ~(bit | (bit << 32)));
+    atomic_and_u64(reinterpret_cast<uint64_t*>(&words[k]->live), ~(bit | (bit << 32)));
   }
 }
 
@@ -979,8 +1013,14 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* _
                             const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  for (uint64_t i = tid; i < WA; i += stride) s.LA[i] = LiveWord{s.rows.Aout[i] | s.rows.Ain[i].bits, 0u};
-  for (uint64_t i = tid; i < WC; i += stride) s.LC[i] = LiveWord{s.rows.Cm[i].bits, 0u};
+  for (uint64_t i = tid; i < WA; i += stride) {
+    const RankWord a = s.rows.Ain[i];
+    s.PA[i] = PeelWord{a.bits, a.rank, s.rows.Aout[i] | a.bits, 0u};
+  }
+  for (uint64_t i = tid; i < WC; i += stride) {
+    const RankWord c = s.rows.Cm[i];
+    s.PC[i] = PeelWord{c.bits, c.rank, c.bits, 0u};
+  }
   for (uint64_t u = tid; u < n; u += stride)  // in-lists are sorted by source id
     icut[u] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(IE, IO[u], IO[u + 1], static_cast<uint32_t>(u)) - IO[u]);
 }

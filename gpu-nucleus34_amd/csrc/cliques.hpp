@@ -21,24 +21,28 @@ struct K4Src {
   // pair gives with its id in one load)
   const uint2* cSlot;
   uint64_t T;
-  // Live rows for the on-demand peel (null elsewhere), word for word beside
-  // (Aout|Ain) and Cm: `live` holds the bits of the triangles not yet retired,
-  // `dying` those of the current round's frontier. A triangle's three bits of
-  // both planes are cleared after the round it dies in, so a dying triangle
-  // enumerates only K4s whose two partners on its lowest vertex's rows are
-  // still alive or dying in the same round, and it drops at once, before any
-  // partner lookup, every K4 one of those two partners owns (dying now with a
-  // lower index).
-  struct LiveWord* LA;
-  struct LiveWord* LC;
+  // Peel rows for the on-demand peel (null elsewhere), word for word beside
+  // (Aout|Ain) and Cm: each word repeats the Ain (or Cm) bits and running rank
+  // next to two planes, `live` (the triangles not yet retired) and `dying`
+  // (the current round's frontier). A triangle's three bits of both planes are
+  // cleared after the round it dies in, so a dying triangle enumerates only
+  // K4s whose two partners on its lowest vertex's rows are still alive or dying
+  // in the same round, and it drops at once, before any partner lookup, every
+  // K4 one of those two partners owns (dying now with a lower index). The
+  // third partner's rank lookup reads the word that holds its planes too, so
+  // its state costs no load of its own.
+  struct PeelWord* PA;
+  struct PeelWord* PC;
   // per vertex u: in-neighbours with a lower vertex id than u (a prefix of
   // N-(u), whose triangles precede u's in the canonical order); on-demand peel only
   const uint32_t* icut;
 };
 
-struct alignas(8) LiveWord {
-  uint32_t live;
-  uint32_t dying;
+struct alignas(16) PeelWord {
+  uint32_t bits;  // Ain (PA) or Cm (PC) bits
+  uint32_t rank;  // their running rank
+  uint32_t live;  // (Aout|Ain) or Cm bits of the triangles not yet retired
+  uint32_t dying; // bits of the current round's frontier
 };
 
 // sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
