@@ -180,6 +180,12 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
              "--workload", workload, "--edges", edges,
              *(["--tuning", json.dumps(tuning)] if tuning else []), "--copy-calibration-mib", str(calib_mib)]
     res = {}
+    # Peel rounds launched one kernel at a time instead of as captured graphs:
+    # the same kernels with the same launch shapes, so the per-dispatch counters
+    # are unchanged. A --kernel-trace run of this library faulted inside
+    # hipGraphLaunch, in the profiler's own code called from the HSA runtime,
+    # so the profiled children keep away from graph launches.
+    env = dict(os.environ, NUC34_NO_GRAPH="1")
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = os.path.join(tmp, counter)
         print(f"[bench] rocprofv3 --pmc {counter} child run", file=sys.stderr, flush=True)
@@ -187,7 +193,7 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
             # the profiler can exit nonzero after writing complete results, so
             # the database decides, not the exit status
             rc = subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "--", *child], timeout=240,
-                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL).returncode
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env).returncode
         except (subprocess.SubprocessError, OSError) as e:
             print(f"traffic: {counter} run failed: {e}", file=sys.stderr)
             return None
