@@ -45,6 +45,8 @@ struct nuc34_ctx {
   DevBuf O, E, src;
   // batch: vertex and oriented-edge boundaries of the graphs of a union
   std::vector<uint64_t> batch_v, batch_e;
+  // per graph: its rounds, when the last peel ran one workgroup per graph
+  std::vector<uint32_t> batch_rounds;
   // triangles and the bit-packed incidence
   bool has_tris = false;
   uint64_t T = 0;
@@ -303,6 +305,7 @@ void reset_stages(nuc34_ctx* c, int from) {
   if (from <= 4) {
     c->kv.release();
     c->has_k = false;
+    c->batch_rounds.clear();
   }
   if (from <= 3) {
     for (DevBuf* b : {&c->koff, &c->lists}) b->release();
@@ -482,6 +485,8 @@ void run_k4_lists(nuc34_ctx* c) {
 
 // Rounds enqueued per host check of the done flag (captured once as a graph).
 constexpr int kRoundsPerLaunch = 32;
+// Most triangles of a batch's graph peeled by one workgroup (by default).
+constexpr uint32_t kGraphPeelMaxTris = 1u << 18;
 
 PeelShape peel_shape(const nuc34_ctx* c) {
   PeelShape sh;
@@ -523,6 +528,49 @@ void run_peel(nuc34_ctx* c, int variant) {
     v.PC = liveC.as<PeelWord>();
     v.icut = icut.as<uint32_t>();
   }
+  // A loaded batch whose graphs are all small: one workgroup per graph, the
+  // whole peel one launch (k_peel_graphs). The graphs' triangle slices come
+  // from offA at the graphs' first edges; workgroups take the largest first.
+  const uint64_t ngraphs = c->batch_e.empty() ? 0 : c->batch_e.size() - 1;
+  std::vector<uint64_t> tfirst;
+  DevBuf gtf, gord, gout;
+  GraphQueue gq{};
+  const uint32_t graph_max = c->tune.graph_peel_max_tris ? c->tune.graph_peel_max_tris : kGraphPeelMaxTris;
+  bool by_graph = ngraphs > 0 && c->tune.batch_peel != 1 && !c->noalloc;
+  if (by_graph) {
+    gtf = c->alloc.get((ngraphs + 1) * 8, "batch triangle offsets");
+    {
+      DevBuf idx = c->alloc.get((ngraphs + 1) * 8, "batch boundaries");
+      NUC34_HIP(hipMemcpyAsync(idx.as<void>(), c->batch_e.data(), (ngraphs + 1) * 8, hipMemcpyHostToDevice, c->s));
+      launch_gather_u64(c->offA.as<uint64_t>(), idx.as<uint64_t>(), gtf.as<uint64_t>(), ngraphs + 1, c->s);
+      tfirst.resize(ngraphs + 1);
+      NUC34_HIP(hipMemcpyAsync(tfirst.data(), gtf.as<void>(), (ngraphs + 1) * 8, hipMemcpyDeviceToHost, c->s));
+      NUC34_HIP(hipStreamSynchronize(c->s));
+    }
+    if (tfirst.front() != 0 || tfirst.back() != T) throw std::logic_error("batch triangle offsets inconsistent");
+    for (uint64_t g = 0; g < ngraphs && by_graph; ++g) {
+      if (tfirst[g + 1] < tfirst[g]) throw std::logic_error("batch triangle offsets not monotone");
+      by_graph = tfirst[g + 1] - tfirst[g] <= graph_max;
+    }
+  }
+  if (by_graph) {
+    std::vector<uint32_t> order(ngraphs);
+    for (uint32_t g = 0; g < ngraphs; ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+      return tfirst[x + 1] - tfirst[x] > tfirst[y + 1] - tfirst[y];
+    });
+    gord = c->alloc.get(ngraphs * 4, "batch peel order");
+    gout = c->alloc.get(ngraphs * 8 + 8, "batch peel rounds and levels");
+    NUC34_HIP(hipMemcpyAsync(gord.as<void>(), order.data(), ngraphs * 4, hipMemcpyHostToDevice, c->s));
+    NUC34_HIP(hipMemsetAsync(gout.as<void>(), 0, ngraphs * 8 + 8, c->s));
+    gq.tri_first = gtf.as<uint64_t>();
+    gq.order = gord.as<uint32_t>();
+    gq.count = static_cast<uint32_t>(ngraphs);
+    gq.rounds = gout.as<uint32_t>();
+    gq.level = reinterpret_cast<int32_t*>(gout.as<uint32_t>() + ngraphs);
+    gq.next = gout.as<uint32_t>() + 2 * ngraphs;
+    gq.error = gq.next + 1;
+  }
   // round parity must equal slot parity: groups of even size
   const int per_launch = c->tune.peel_rounds_per_launch ? static_cast<int>(c->tune.peel_rounds_per_launch & ~1u)
                                                         : kRoundsPerLaunch;
@@ -542,6 +590,28 @@ void run_peel(nuc34_ctx* c, int variant) {
     if (variant != NUC34_DIRECT)
       launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(), c->s);
   });
+  if (by_graph) {
+    c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
+      ktime(c, K_PEEL, 1, [&] {
+        launch_peel_graphs(pb, v, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), gq, shape, variant != NUC34_DIRECT,
+                           c->s);
+      });
+    });
+    std::vector<uint32_t> out(2 * ngraphs + 2);
+    NUC34_HIP(hipMemcpyAsync(out.data(), gout.as<void>(), out.size() * 4, hipMemcpyDeviceToHost, c->s));
+    NUC34_HIP(hipStreamSynchronize(c->s));
+    if (out[2 * ngraphs + 1]) throw std::logic_error("peel: inconsistent state (code " + std::to_string(out[2 * ngraphs + 1]) + ")");
+    c->batch_rounds.assign(out.begin(), out.begin() + ngraphs);
+    uint32_t rmax = 0;
+    int32_t lmax = 0;
+    for (uint64_t g = 0; g < ngraphs; ++g) {
+      rmax = std::max(rmax, out[g]);
+      lmax = std::max(lmax, static_cast<int32_t>(out[ngraphs + g]));
+    }
+    h.round = rmax;
+    h.level = lmax;
+    h.peeled = T;
+  } else {
   c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
     struct Captured {  // destroyed on every way out, errors included
       hipStream_t s = nullptr;
@@ -625,6 +695,7 @@ void run_peel(nuc34_ctx* c, int variant) {
     });
     c->kt[K_PEEL].launches += launches * static_cast<uint64_t>(per_launch);
   });
+  }
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
   c->st.rounds = h.round;
   c->st.k_max = h.level;
@@ -1067,6 +1138,15 @@ int nuc34_batch_slices(nuc34_ctx* c, uint64_t* edge_first, uint64_t* tri_first) 
   });
 }
 
+int nuc34_batch_rounds(nuc34_ctx* c, uint32_t* rounds) {
+  if (!c || !rounds) return fail(c, NUC34_E_INVALID, "null argument");
+  if (c->batch_v.empty()) return fail(c, NUC34_E_STATE, "no batch loaded");
+  if (!c->has_k || c->batch_rounds.size() + 1 != c->batch_e.size())
+    return fail(c, NUC34_E_STATE, "the last peel did not run one workgroup per graph");
+  std::copy(c->batch_rounds.begin(), c->batch_rounds.end(), rounds);
+  return NUC34_OK;
+}
+
 int nuc34_graph_dims(const nuc34_ctx* c, uint32_t* n, uint64_t* m) {
   if (!c || !c->has_graph) return NUC34_E_STATE;
   if (n) *n = c->n;
@@ -1163,13 +1243,15 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
       if (c->T) run_support(c);
       else c->has_sup = true;
     }
-    // AUTO: the on-demand peel enumerates each dying triangle's K4s from the
-    // ranked rows at O(1) per K4 and skips K4s destroyed in earlier rounds, so
-    // it needs no 48 bytes per K4 and ran faster than the direct variant on
-    // every measured workload (one graph and many, 10^4 to 10^10 K4s); direct
-    // runs only when asked for.
+    // AUTO for one graph: the on-demand peel enumerates each dying triangle's
+    // K4s from the ranked rows at O(1) per K4 and skips K4s destroyed in
+    // earlier rounds, so it needs no 48 bytes per K4 and ran faster than the
+    // direct variant on every measured single graph (10^4 to 10^10 K4s). AUTO
+    // for a batch of graphs is AUTO_FIT: the union's graphs are small, their
+    // lists cheap to build, and the direct peel one workgroup per graph beat
+    // the on-demand one (512-graph mix: 22.6 against 26.1 ms).
     int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
-    if (variant == NUC34_AUTO_FIT) {
+    if (variant == NUC34_AUTO_FIT || (variant == NUC34_AUTO && c->batch_e.size() > 2)) {
       // the direct variant's extra memory: the lists (48 bytes per K4) and
       // their offsets, plus the peel's O(T) state both variants hold
       const uint64_t need = 48 * c->Q + 8 * (c->T + 1) + 20 * c->T;
@@ -1266,6 +1348,7 @@ int nuc34_set_tuning(nuc34_ctx* c, const nuc34_tuning* t) {
     return fail(c, NUC34_E_INVALID, "grid above 2^20 workgroups");
   if (t->tri_tile_edges && t->tri_tile_edges != 16 && t->tri_tile_edges != 32 && t->tri_tile_edges != 64)
     return fail(c, NUC34_E_INVALID, "tri_tile_edges must be 16, 32 or 64");
+  if (t->batch_peel > 2) return fail(c, NUC34_E_INVALID, "batch_peel must be 0, 1 or 2");
   c->tune = *t;
   c->tune.peel_rounds_per_launch &= ~1u;
   return NUC34_OK;
@@ -1280,15 +1363,18 @@ int nuc34_get_tuning(const nuc34_ctx* c, nuc34_tuning* t) {
   if (!t->k4_list_grid_cap) t->k4_list_grid_cap = 1u << 15;
   if (!t->tri_tile_edges) t->tri_tile_edges = 64;
   if (!t->peel_finish_grid) t->peel_finish_grid = PeelShape{}.finish_grid;
+  if (!t->batch_peel) t->batch_peel = 2;
+  if (!t->graph_peel_max_tris) t->graph_peel_max_tris = kGraphPeelMaxTris;
   return NUC34_OK;
 }
 
 int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_ms) {
   if (!c || !c->has_graph) return fail(c, NUC34_E_STATE, "no graph loaded");
-  const int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
-  if (v != NUC34_DIRECT && v != NUC34_ONDEMAND) return fail(c, NUC34_E_INVALID, "unknown variant");
+  if (variant != NUC34_DIRECT && variant != NUC34_ONDEMAND && variant != NUC34_AUTO && variant != NUC34_AUTO_FIT)
+    return fail(c, NUC34_E_INVALID, "unknown variant");
   nuc34_tuning cur;
   nuc34_get_tuning(c, &cur);
+  int v = variant;  // the variant that runs (AUTO and AUTO_FIT resolve in the first decomposition)
   // a candidate's time: the median of three whole decompositions
   auto time_of = [&](const nuc34_tuning& t, double* ms) -> int {
     c->tune = t;
@@ -1299,6 +1385,7 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
       if (rc == NUC34_OK) rc = nuc34_decompose(c, v, 1, &st);
       if (rc != NUC34_OK) return rc;
       r_ms = st.ms_total;
+      v = st.variant;
     }
     std::sort(runs, runs + 3);
     *ms = runs[1];
@@ -1310,15 +1397,18 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
   struct Knob {
     uint32_t nuc34_tuning::*field;
     std::vector<uint32_t> values;
-    bool direct_only, ondemand_only;
+    bool direct_only, ondemand_only, batch_only;
   };
+  // the peel schedule of a batch first (rounds shared by the union, or one
+  // workgroup per graph), then the launch shapes under it
   const Knob knobs[] = {
-      {&nuc34_tuning::peel_debit_grid, {1024, 2048, 4096, 8192}, false, false},
-      {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, false},
-      {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false},
-      {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false},
-      {&nuc34_tuning::tri_tile_edges, {16, 32, 64}, false, false},
-      {&nuc34_tuning::peel_finish_grid, {256, 512, 1024}, false, false},
+      {&nuc34_tuning::batch_peel, {1, 2}, false, false, true},
+      {&nuc34_tuning::peel_debit_grid, {1024, 2048, 4096, 8192}, false, false, false},
+      {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, false, false},
+      {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false, false},
+      {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false, false},
+      {&nuc34_tuning::tri_tile_edges, {16, 32, 64}, false, false, false},
+      {&nuc34_tuning::peel_finish_grid, {256, 512, 1024}, false, false, false},
   };
   // coordinate descent, repeated until a pass changes nothing (at most three
   // passes); a candidate replaces the current shape only when it is faster by
@@ -1326,7 +1416,9 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
   for (int pass = 0; pass < 3; ++pass) {
     bool changed = false;
     for (const Knob& k : knobs) {
-      if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND)) continue;
+      if ((k.direct_only && v != NUC34_DIRECT) || (k.ondemand_only && v != NUC34_ONDEMAND) ||
+          (k.batch_only && c->batch_v.empty()))
+        continue;
       for (uint32_t val : k.values) {
         if (cur.*k.field == val) continue;
         nuc34_tuning t = cur;

@@ -905,22 +905,14 @@ __device__ __forceinline__ uint32_t find_slot_u64(const uint64_t* pre, uint64_t 
 // K4 list entries over its lanes, two entries per lane and pass, so a short
 // list does not leave most of a wave idle. Both entries' loads, then both
 // entries' state reads, are issued before either is waited on.
-__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
-                                                             const uint64_t* __restrict__ koff,
-                                                             const uint32_t* __restrict__ lists, uint32_t tile_cap) {
-  __shared__ ListTile lds[kWPB];
-  const bool done = in->done;
-  const int32_t level = round_level(in);
-  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
-  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
-  uint32_t* ncount = &in->nnext;
-  const uint32_t n = done ? 0u : in->ncur + in->nspec;
-  const unsigned w = threadIdx.x / kWave;
+// The direct debit of the n dying triangles cur[0, n): the waves numbered
+// wave, wave + nwaves, ... take G of them at a time.
+__device__ __forceinline__ void debit_lists(const PeelBufs& p, const uint32_t* __restrict__ cur, uint32_t n,
+                                            uint64_t wave, uint64_t nwaves, uint32_t G, ListTile& L,
+                                            const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists,
+                                            int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
   const unsigned lane = lane_id();
-  ListTile& L = lds[w];
-  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
-  const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
+  for (uint64_t tile = wave; tile * G < n; tile += nwaves) {
     const uint64_t i = tile * G + lane;
     uint32_t t = 0;
     uint64_t b = 0, len = 0;
@@ -962,6 +954,43 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
     }
     wave_sync();
   }
+}
+
+// The on-demand debit of the n dying triangles cur[0, n): the waves numbered
+// wave, wave + nwaves, ... take G of them at a time and enumerate their K4s
+// from the live bit rows, flattened over the lanes.
+__device__ __forceinline__ void debit_bits(const PeelBufs& p, const K4Src& s, const uint32_t* __restrict__ cur,
+                                           uint32_t n, uint64_t wave, uint64_t nwaves, uint32_t G, TileLds& L,
+                                           int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+  const unsigned lane = lane_id();
+  for (uint64_t tile = wave; tile * G < n; tile += nwaves) {
+    const uint64_t i = tile * G + lane;
+    const bool valid = lane < G && i < n;
+    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, L,
+                         [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3], uint32_t die) {
+                           destroy_k4_live(tt, q, die, p, level, nxt, ncount);
+                         });
+  }
+}
+
+// Direct debit: a wave takes tile_width() dying triangles and flattens their
+// K4 list entries over its lanes, two entries per lane and pass, so a short
+// list does not leave most of a wave idle. Both entries' loads, then both
+// entries' state reads, are issued before either is waited on.
+__global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl* in, PeelCtl* out,
+                                                             const uint64_t* __restrict__ koff,
+                                                             const uint32_t* __restrict__ lists, uint32_t tile_cap) {
+  __shared__ ListTile lds[kWPB];
+  const bool done = in->done;
+  const int32_t level = round_level(in);
+  const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
+  uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
+  const uint32_t n = done ? 0u : in->ncur + in->nspec;
+  const unsigned w = threadIdx.x / kWave;
+  const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
+  const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
+  debit_lists(p, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], koff, lists, level, nxt,
+              &in->nnext);
   prepare_out_slot(out, done);
 }
 
@@ -974,21 +1003,187 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   const int32_t level = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
-  uint32_t* ncount = &in->nnext;
   const uint32_t n = done ? 0u : in->ncur + in->nspec;
   const unsigned w = threadIdx.x / kWave;
-  const unsigned lane = lane_id();
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kWPB + w; tile * G < n; tile += nw) {
-    const uint64_t i = tile * G + lane;
-    const bool valid = lane < G && i < n;
-    k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, lds[w],
-                         [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3], uint32_t die) {
-                           destroy_k4_live(tt, q, die, p, level, nxt, ncount);
-                         });
-  }
+  debit_bits(p, s, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], level, nxt, &in->nnext);
   prepare_out_slot(out, done);
+}
+
+// ---- one workgroup per graph ---------------------------------------------------
+//
+// The graphs of a batch share no triangle and no K4, so each one can be peeled
+// by itself. k_peel_graphs peels a whole batch in one launch: a workgroup
+// takes graphs from a queue, largest first, and runs one graph's rounds with
+// workgroup barriers between the debit and the state advance. No graph waits
+// for another's rounds, and nothing returns to the host between rounds. The
+// rounds are those of the graph decomposed alone, with the same level rule,
+// collection and ownership as k_peel_debit_* + k_peel_finish:
+// - when the frontier is empty, level = max(level, min alive support) and the
+//   alive triangles with support <= level form the frontier (their k);
+// - a survivor whose support falls from level+1 to level is pushed to the next
+//   round (its k = level).
+// A graph with no K4 gets k = 0 and no round, as its own decomposition does.
+// Every round retires at least one triangle, so a graph ends after at most
+// its triangle count of rounds.
+
+// The 2-bit fields of state word i that hold triangles of [t0, t1) (low bit
+// of each field; the word overlaps the range).
+__device__ __forceinline__ uint32_t field_mask(uint64_t i, uint64_t t0, uint64_t t1) {
+  const uint64_t lo = 16 * i;
+  uint32_t m = 0x55555555u;
+  if (t0 > lo) m &= 0x55555555u << (2 * (t0 - lo));
+  if (t1 < lo + 16) m &= (1u << (2 * (t1 - lo))) - 1u;
+  return m;
+}
+
+// Workgroup of the per-graph peel: 16 waves, so that the largest graphs of a
+// batch (the launch's critical path) get a CU's worth of waves.
+constexpr int kGraphBlock = 1024;
+constexpr int kGraphWaves = kGraphBlock / kWave;
+
+template <bool LIVE>
+__global__ __launch_bounds__(kGraphBlock) void k_peel_graphs(PeelBufs p, K4Src s, const uint64_t* __restrict__ koff,
+                                                             const uint32_t* __restrict__ lists, GraphQueue q,
+                                                             uint32_t tile_cap) {
+  __shared__ ListTile lt[LIVE ? 1 : kGraphWaves];
+  __shared__ TileLds tl[LIVE ? kGraphWaves : 1];
+  __shared__ uint32_t s_g, s_n, s_nn, s_min, s_max;
+  const unsigned w = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  for (;;) {
+    if (threadIdx.x == 0) s_g = atomic_add_u32(q.next, 1u);
+    __syncthreads();
+    const uint32_t gi = s_g;
+    if (gi >= q.count) return;
+    const uint32_t g = q.order[gi];
+    const uint64_t t0 = q.tri_first[g], t1 = q.tri_first[g + 1];
+    const uint64_t w0 = t0 / 16, w1 = (t1 + 15) / 16;
+    uint32_t* cur = p.buf0 + t0;
+    uint32_t* nxt = p.buf1 + t0;
+    int32_t level = 0;
+    uint32_t rounds = 0, n = 0;
+    bool first = true;
+    for (;;) {
+      if (n == 0) {
+        // min alive support of the graph (and the first time the max: no K4 at all?)
+        if (threadIdx.x == 0) {
+          s_min = kNoMin;
+          s_max = 0;
+        }
+        __syncthreads();
+        uint32_t mn = kNoMin, mx = 0;
+        for (uint64_t i = w0 + threadIdx.x; i < w1; i += kGraphBlock) {
+          const uint32_t al = alive_fields(p.st[i]) & field_mask(i, t0, t1);
+          if (!al) continue;
+          int32_t d[16];
+          load_word_dd(p.dd, i, d);
+#pragma unroll
+          for (int f = 0; f < 16; ++f) {
+            if (!((al >> (2 * f)) & 1u)) continue;
+            const uint32_t v = static_cast<uint32_t>(d[f] < 0 ? 0 : d[f]);
+            mn = min(mn, v);
+            mx = max(mx, v);
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          mn = min(mn, static_cast<uint32_t>(__shfl_xor(mn, off)));
+          mx = max(mx, static_cast<uint32_t>(__shfl_xor(mx, off)));
+        }
+        if (lane == 0) {
+          atomicMin(&s_min, mn);
+          atomicMax(&s_max, mx);
+        }
+        __syncthreads();
+        const uint32_t m = s_min;
+        if (m == kNoMin) break;  // every triangle of the graph has its k
+        if (first && s_max == 0) {
+          for (uint64_t t = t0 + threadIdx.x; t < t1; t += kGraphBlock) {
+            p.kv[t] = 0;
+            atomic_or_u32(&p.st[t >> 4], kGone << ((t & 15u) * 2u));
+          }
+          break;
+        }
+        first = false;
+        level = max(level, static_cast<int32_t>(m));
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        // the alive triangles with support <= level; every lane runs every
+        // pass (wave prefix sums), one LDS append per wave and pass
+        for (uint64_t b = w0; b < w1; b += kGraphBlock) {
+          const uint64_t i = b + threadIdx.x;
+          uint32_t die = 0;
+          if (i < w1) {
+            const uint32_t al = alive_fields(p.st[i]) & field_mask(i, t0, t1);
+            if (al) {
+              int32_t d[16];
+              load_word_dd(p.dd, i, d);
+#pragma unroll
+              for (int f = 0; f < 16; ++f)
+                if (((al >> (2 * f)) & 1u) && d[f] <= level) die |= 1u << f;
+            }
+          }
+          const uint32_t nd = static_cast<uint32_t>(__popc(die));
+          const uint32_t inc = wave_inclusive_sum(nd);
+          const uint32_t tot = __shfl(inc, kWave - 1);
+          uint32_t base = 0;
+          if (lane == kWave - 1 && tot) base = atomicAdd(&s_n, tot);
+          base = __shfl(base, kWave - 1);
+          if (die) {
+            uint32_t o = base + inc - nd, set = 0;
+            for (uint32_t d = die; d; d &= d - 1u) {
+              const uint32_t f = static_cast<uint32_t>(__builtin_ctz(d));
+              const uint32_t t = static_cast<uint32_t>(16 * i + f);
+              p.kv[t] = level;
+              cur[o++] = t;
+              set |= kCur << (2 * f);
+              if constexpr (LIVE) mark_dying(s, t);
+            }
+            atomic_or_u32(&p.st[i], set);  // 00 -> 01
+          }
+        }
+        __syncthreads();
+        n = s_n;
+        if (n == 0) {  // cannot happen: the min alive support is at most level
+          if (threadIdx.x == 0) atomic_or_u32(q.error, 1u);
+          break;
+        }
+      }
+      if (++rounds > t1 - t0) {  // every round retires a triangle
+        if (threadIdx.x == 0) atomic_or_u32(q.error, 2u);
+        break;
+      }
+      if (threadIdx.x == 0) s_nn = 0;
+      __syncthreads();
+      const uint32_t G = min(tile_width(n, kGraphWaves), tile_cap);
+      if constexpr (LIVE) debit_bits(p, s, cur, n, w, kGraphWaves, G, tl[w], level, nxt, &s_nn);
+      else debit_lists(p, cur, n, w, kGraphWaves, G, lt[w], koff, lists, level, nxt, &s_nn);
+      __syncthreads();
+      const uint32_t nn = s_nn;
+      for (uint32_t i = threadIdx.x; i < n; i += kGraphBlock) {
+        const uint32_t t = cur[i];
+        atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
+        if constexpr (LIVE) unlink_tri(s, t);
+      }
+      for (uint32_t i = threadIdx.x; i < nn; i += kGraphBlock) {
+        const uint32_t t = nxt[i];
+        atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
+        if constexpr (LIVE) mark_dying(s, t);
+      }
+      __syncthreads();
+      uint32_t* const tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+      n = nn;
+    }
+    if (threadIdx.x == 0) {
+      q.rounds[g] = rounds;
+      q.level[g] = level;
+    }
+    __syncthreads();  // s_g is rewritten next
+  }
 }
 
 // Sum of the residual supports dd[0..T) into *total (which starts at zero).
@@ -1091,6 +1286,18 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
   hipLaunchKernelGGL(k_peel_debit_lists, dim3(shape.debit_grid), dim3(kBlock), 0, s, p, in, out, koff, lists,
                      static_cast<uint32_t>(shape.tile_cap ? shape.tile_cap : 1));
   hipLaunchKernelGGL(k_peel_finish<false>, dim3(shape.finish_grid), dim3(kBlock), 0, s, p, in, out, K4Src{});
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_peel_graphs(const PeelBufs& p, const K4Src& src, const uint64_t* koff, const uint32_t* lists,
+                        const GraphQueue& q, const PeelShape& shape, bool live, hipStream_t s) {
+  if (!q.count) return;
+  const unsigned grid = q.count < shape.debit_grid ? q.count : shape.debit_grid;
+  const uint32_t cap = static_cast<uint32_t>(shape.tile_cap ? shape.tile_cap : 1);
+  if (live)
+    hipLaunchKernelGGL(k_peel_graphs<true>, dim3(grid), dim3(kGraphBlock), 0, s, p, src, koff, lists, q, cap);
+  else
+    hipLaunchKernelGGL(k_peel_graphs<false>, dim3(grid), dim3(kGraphBlock), 0, s, p, src, koff, lists, q, cap);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -126,6 +126,25 @@ void launch_peel_round_direct(const PeelBufs& p, int slot, const uint64_t* koff,
                               const PeelShape& shape, hipStream_t s);
 void launch_peel_round_ondemand(const PeelBufs& p, int slot, const K4Src& src, const PeelShape& shape, hipStream_t s);
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s);
+
+// A batch peeled one workgroup per graph (k_peel_graphs): each workgroup takes
+// the next graph of `order` and runs all of that graph's rounds, with the
+// same rounds, k and ownership as the graph decomposed alone.
+struct GraphQueue {
+  const uint64_t* tri_first;  // count + 1 triangle offsets of the batch's graphs
+  const uint32_t* order;      // the graphs in the order workgroups take them
+  uint32_t count;             // graphs in `order`
+  uint32_t* next;             // queue position, 0 at launch
+  uint32_t* rounds;           // per graph: its rounds (with a non-empty frontier)
+  int32_t* level;             // per graph: its last level, its K_max
+  uint32_t* error;            // nonzero: inconsistent state (0 at launch)
+};
+// One launch for the whole peel of the graphs in q (after launch_peel_init,
+// and with `live`, launch_live_rows): the direct debit from koff/lists, or
+// with `live` the on-demand debit from src's live rows. Up to
+// shape.debit_grid workgroups; shape.tile_cap dying triangles per wave.
+void launch_peel_graphs(const PeelBufs& p, const K4Src& src, const uint64_t* koff, const uint32_t* lists,
+                        const GraphQueue& q, const PeelShape& shape, bool live, hipStream_t s);
 // Sum of the residual supports after the peel into *total (zeroed by the
 // caller): the peel applied 4Q minus this many support decrements.
 void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s);

@@ -61,7 +61,8 @@ class DeviceRates(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("peel_debit_grid", C.c_uint32), ("peel_tile_cap", C.c_uint32),
                 ("peel_rounds_per_launch", C.c_uint32), ("k4_list_grid_cap", C.c_uint32),
-                ("tri_tile_edges", C.c_uint32), ("peel_finish_grid", C.c_uint32)]
+                ("tri_tile_edges", C.c_uint32), ("peel_finish_grid", C.c_uint32),
+                ("batch_peel", C.c_uint32), ("graph_peel_max_tris", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -126,6 +127,7 @@ def _load() -> C.CDLL:
         "nuc34_load_batch_ptrs": (C.c_int, [vp, C.c_uint32, u32p, u64p, C.POINTER(u32p), C.POINTER(u32p), C.c_int]),
         "nuc34_batch_graph": (C.c_int, [vp, C.c_uint32, u64p, u64p, u64p, u64p]),
         "nuc34_batch_slices": (C.c_int, [vp, u64p, u64p]),
+        "nuc34_batch_rounds": (C.c_int, [vp, u32p]),
         "nuc34_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "nuc34_measure_device": (C.c_int, [C.c_int, C.POINTER(DeviceRates)]),
         "nuc34_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
@@ -329,6 +331,13 @@ class Context:
         p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))
         self._check(_lib.nuc34_batch_slices(self._h, p(ef), p(tf)))
         return ef, tf
+
+    def batch_rounds(self) -> np.ndarray:
+        """Peel rounds of every graph of the batch, when the last decomposition
+        peeled it one workgroup per graph (each equals the graph's own rounds)."""
+        r = np.empty(max(self._batch, 1), dtype=np.uint32)
+        self._check(_lib.nuc34_batch_rounds(self._h, _u32(r)))
+        return r[:self._batch]
 
     def dims(self):
         n, m = C.c_uint32(), C.c_uint64()

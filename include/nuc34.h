@@ -150,6 +150,13 @@ int nuc34_batch_graph(nuc34_ctx* ctx, uint32_t g, uint64_t* vertex_first, uint64
 /* All graphs' boundaries at once: edge_first and tri_first receive count + 1
  * entries each (the last one is the union's total); either may be NULL. */
 int nuc34_batch_slices(nuc34_ctx* ctx, uint64_t* edge_first, uint64_t* tri_first);
+/* The peel rounds of every graph of the batch (count entries) from the last
+ * decomposition, when it peeled the batch one workgroup per graph (tuning
+ * batch_peel); each equals the rounds of that graph decomposed alone, and the
+ * union's stats.rounds is their maximum. NUC34_E_STATE when the last peel
+ * shared its rounds across the union. No counterpart in the reference, which
+ * decomposes one graph per process (nucleus/cuNucleus34.cu:144-175). */
+int nuc34_batch_rounds(nuc34_ctx* ctx, uint32_t* rounds);
 int nuc34_copy_oriented(nuc34_ctx* ctx, uint32_t* O, uint32_t* E);
 
 /* Triangles in the reference's canonical (a,b) order; `parts` >= 1 splits the
@@ -169,8 +176,9 @@ int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
 
 /* Runs the missing stages and the peel. variant: NUC34_DIRECT (per-triangle
  * K4 lists in HBM, 48 bytes per K4), NUC34_ONDEMAND (K4s re-enumerated from the
- * bit-packed incidence as triangles die, O(T) memory), NUC34_AUTO (on-demand:
- * never slower and never larger on the measured workloads) or NUC34_AUTO_FIT
+ * bit-packed incidence as triangles die, O(T) memory), NUC34_AUTO (one graph:
+ * on demand, never slower and never larger on the measured single graphs; a
+ * batch of two or more graphs: as NUC34_AUTO_FIT) or NUC34_AUTO_FIT
  * (direct when, with Q known from the support count, its lists fit what is
  * left of the context's budget, or of free device memory without one, with a
  * quarter to spare; on demand otherwise: the reference's own choice between
@@ -195,16 +203,22 @@ typedef struct nuc34_tuning {
   uint32_t k4_list_grid_cap;       /* most workgroups of the direct K4 list build (32768) */
   uint32_t tri_tile_edges;         /* oriented edges per wave tile of the triangle intersection: 16, 32, 64 (64) */
   uint32_t peel_finish_grid;       /* workgroups of a peel round's finish kernel (512) */
+  uint32_t batch_peel;             /* a loaded batch: 1 = peel rounds shared by the whole union; 2 = one
+                                      workgroup per graph, the whole peel one launch, when no graph has
+                                      more than graph_peel_max_tris triangles; 0 = 2 (0) */
+  uint32_t graph_peel_max_tris;    /* most triangles of a graph peeled by one workgroup (1 << 18) */
 } nuc34_tuning;
 
 int nuc34_set_tuning(nuc34_ctx* ctx, const nuc34_tuning* tuning);
 int nuc34_get_tuning(const nuc34_ctx* ctx, nuc34_tuning* tuning);
-/* Times candidate launch shapes on the loaded graph, one parameter at a time
- * from the current setting, in passes until a pass changes nothing (at most
- * three; each candidate: the median of three whole decompositions in
- * `variant`, adopted when more than 1% faster), keeps the fastest in the
- * context and returns it with its decomposition time in milliseconds. The reference launches with
- * fixed shapes (MAX_THRD_BLK, nucleus/utils.cuh:29); no counterpart there. */
+/* Times candidate schedules and launch shapes on the loaded graph, one
+ * parameter at a time from the current setting (for a batch, the peel schedule
+ * first: shared rounds or one workgroup per graph), in passes until a pass
+ * changes nothing (at most three; each candidate: the median of three whole
+ * decompositions in `variant`, adopted when more than 1% faster), keeps the
+ * fastest in the context and returns it with its decomposition time in
+ * milliseconds. The reference launches with fixed shapes (MAX_THRD_BLK,
+ * nucleus/utils.cuh:29); no counterpart there. */
 int nuc34_autotune(nuc34_ctx* ctx, int variant, nuc34_tuning* best, double* best_ms);
 
 /* Device description as "key=value\n" lines (name, arch, memory, CUs, ...). */
@@ -257,14 +271,17 @@ int nuc34_decompose_noalloc(int device, const uint32_t* O, const uint32_t* E, ui
  * threads, each with its own context and stream on the device, take them in
  * arrival order: `small_workers` small-job workers and `large_workers`
  * large-job workers. A free small-job worker packs every queued job that fits
- * its caps into one disjoint-union batch (one pass of every kernel, shared peel
- * rounds); with more than one, jobs that arrive while a batch is on the device
+ * its caps into one disjoint-union batch (one pass of every kernel and one
+ * peel); with more than one, jobs that arrive while a batch is on the device
  * start at once on another worker, their work overlapping the running batch
- * (continuous batching), instead of waiting for the batch's slowest graph. The
+ * (continuous batching), instead of waiting for the batch's slowest graph; a
+ * batch of small graphs is peeled one workgroup per graph (tuning batch_peel).
+ * The
  * large-job lane runs graphs at or above `large_edges` edge entries or
  * `large_wedges` wedges one at a time, so they never hold small jobs back.
  * Admission is by HBM footprint: each lane holds its share of the budget, a batch runs
- * in the configured variant (NUC34_AUTO: on demand, O(T) memory), and a batch
+ * in the configured variant (NUC34_AUTO: on demand for a batch of one graph,
+ * direct when its lists fit for a batch of several), and a batch
  * that runs out of its lane's budget is split in halves and retried.
  */
 typedef struct nuc34_server nuc34_server;

@@ -300,21 +300,89 @@ def test_full_batch_512_graphs_vs_oracle():
     support and k equals the oracle's decomposition of that graph alone."""
     gs = [nuc34.gen_batch_graph(3, i) for i in range(512)]
     refs = [oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 16) for g in gs]
-    for variant in ("ondemand", "direct"):
-        print(f"[batch512] {variant}", flush=True)
+    for variant, batch_peel in itertools.product(("ondemand", "direct"), (2, 1)):
+        print(f"[batch512] {variant} batch_peel={batch_peel}", flush=True)
         with nuc34.Context() as ctx:
+            ctx.set_tuning(batch_peel=batch_peel)
             ctx.load_batch(gs)
             st = ctx.decompose(variant)
             tris, k, sup = ctx.copy_triangles(), ctx.copy_k(), ctx.copy_support()
             ef, tf = ctx.batch_slices()
+            rounds = ctx.batch_rounds() if batch_peel == 2 else None
         assert st["unpeeled"] == 0 and st["triangles"] == sum(r["T"] for r in refs)
         assert st["four_cliques"] == sum(r["Q"] for r in refs)
-        assert st["rounds"] >= max(r["rounds"] for r in refs)
+        assert st["k_max"] == max(r["K_max"] for r in refs)
+        if batch_peel == 2:  # one workgroup per graph: every graph's own rounds
+            assert [int(x) for x in rounds] == [r["rounds"] for r in refs], variant
+            assert st["rounds"] == max(r["rounds"] for r in refs)
+        else:
+            assert st["rounds"] >= max(r["rounds"] for r in refs)
         for j, r in enumerate(refs):
             sl = slice(int(tf[j]), int(tf[j + 1]))
             assert np.array_equal(tris[sl] - np.uint32(ef[j]), r["tris"]), (variant, j)
             assert np.array_equal(sup[sl], r["support"]), (variant, j)
             assert np.array_equal(k[sl], r["k"]), (variant, j)
+
+
+def _pairs(edges):
+    a = np.array([x for x, _ in edges], dtype=np.uint32)
+    b = np.array([y for _, y in edges], dtype=np.uint32)
+    return nuc34.graph_from_pairs(a, b)
+
+
+@pytest.mark.parametrize("variant", ["direct", "ondemand"])
+def test_batch_peel_one_workgroup_per_graph(variant):
+    """A batch peeled one workgroup per graph (the whole peel one launch) gives
+    the k of the union's shared rounds, and every graph's rounds equal the
+    oracle's for that graph alone. Graphs without triangles (a star, no edges),
+    without K4s (a wheel) and single cliques (one round, k = n - 3) sit between
+    mixed-batch graphs, so graph slices start and end inside state words."""
+    star = _pairs([(0, i) for i in range(1, 9)])
+    wheel = _pairs([(0, i) for i in range(1, 9)] + [(i, i % 8 + 1) for i in range(1, 9)])
+    k7 = _pairs(list(itertools.combinations(range(7), 2)))
+    k12 = _pairs(list(itertools.combinations(range(12), 2)))
+    empty = _pairs([(0, 0)])
+    mix = [nuc34.gen_batch_graph(3, i) for i in range(20)]
+    gs = [mix[0], star, mix[1], wheel, k7, mix[2], empty, k12] + mix[3:] + [wheel, k7]
+    refs = [oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 4) for g in gs]
+    out = {}
+    for batch_peel in (2, 1):
+        with nuc34.Context() as ctx:
+            ctx.set_tuning(batch_peel=batch_peel)
+            ctx.load_batch(gs)
+            st = ctx.decompose(variant)
+            out[batch_peel] = (ctx.copy_k(), st, ctx.batch_slices()[1],
+                               ctx.batch_rounds() if batch_peel == 2 else None)
+            if batch_peel == 1:
+                with pytest.raises(nuc34.Nuc34Error):
+                    ctx.batch_rounds()
+    k2, st2, tf, rounds = out[2]
+    k1, st1, _, _ = out[1]
+    assert np.array_equal(k2, k1) and st2["unpeeled"] == 0 and st2["k_max"] == st1["k_max"]
+    assert [int(x) for x in rounds] == [r["rounds"] for r in refs]
+    assert st2["rounds"] == max(r["rounds"] for r in refs)
+    for j, r in enumerate(refs):
+        assert np.array_equal(k2[int(tf[j]):int(tf[j + 1])], r["k"]), j
+    assert rounds[7] == 1 and (k2[int(tf[7]):int(tf[8])] == 9).all()  # gs[7] is K_12
+
+
+def test_batch_peel_falls_back_to_shared_rounds_for_large_graphs():
+    """A graph above graph_peel_max_tris keeps the batch on shared rounds (no
+    per-graph rounds), with the same k; raising the limit peels it by graph."""
+    gs = [nuc34.gen_batch_graph(3, i) for i in range(6)]
+    with nuc34.Context() as ctx:
+        ctx.load_batch(gs)
+        ctx.triangles()
+        tc = max(ctx.batch_graph(g)["tri_count"] for g in range(len(gs)))
+        ctx.set_tuning(graph_peel_max_tris=max(tc - 1, 1))
+        ctx.decompose("direct")
+        k_shared = ctx.copy_k()
+        with pytest.raises(nuc34.Nuc34Error):
+            ctx.batch_rounds()
+        ctx.set_tuning(graph_peel_max_tris=tc)
+        ctx.reset()
+        ctx.decompose("direct")
+        assert np.array_equal(ctx.copy_k(), k_shared) and len(ctx.batch_rounds()) == len(gs)
 
 
 def test_fourclique_copy_fits_beside_the_direct_peel():
