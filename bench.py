@@ -87,11 +87,11 @@ WORKLOADS = {
 # prints the tuner's choice); shapes change speed, never results.
 TUNED = {
     "rmat20k30": {"peel_debit_grid": 8192, "peel_tile_cap": 8, "peel_rounds_per_launch": 64, "tri_tile_edges": 16,
-                  "peel_finish_grid": 1024},
+                  "peel_finish_grid": 1024, "tri_schedule": 1},
     "rmat16": {"peel_debit_grid": 4096, "peel_tile_cap": 16, "peel_rounds_per_launch": 32, "k4_list_grid_cap": 8192,
-               "tri_tile_edges": 16, "peel_finish_grid": 512},
+               "tri_tile_edges": 64, "peel_finish_grid": 512, "tri_schedule": 1},
     "batch": {"peel_debit_grid": 2048, "peel_tile_cap": 32, "peel_rounds_per_launch": 16, "k4_list_grid_cap": 32768,
-              "tri_tile_edges": 64, "peel_finish_grid": 512},
+              "tri_tile_edges": 64, "peel_finish_grid": 512, "batch_peel": 2, "tri_schedule": 0},
 }
 
 
@@ -132,11 +132,11 @@ def algorithmic_bytes(kernel: str, g: dict, st: dict) -> float:
 # Kernels that make up each timed group (the per-kernel event timers of the
 # library time these groups; the peel group is one captured graph of rounds).
 GROUP_KERNELS = {
-    "tri_enum_bits": ("k_tri_enum<false>",),
-    "tri_enum_emit": ("k_tri_enum<true>",),
+    "tri_enum_bits": ("k_tri_enum<false>", "k_tri_vertex_wave<false>", "k_tri_vertex_group<false"),
+    "tri_enum_emit": ("k_tri_enum<true>", "k_tri_vertex_wave<true>", "k_tri_vertex_group<true"),
     "support_bits": ("k_support_bits",),
     "k4_lists": ("k_k4_lists",),
-    "peel_rounds": ("k_peel_collect", "k_peel_debit", "k_peel_finish"),
+    "peel_rounds": ("k_peel_collect", "k_peel_debit", "k_peel_finish", "k_peel_graphs"),
 }
 
 

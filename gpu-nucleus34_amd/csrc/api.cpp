@@ -55,6 +55,7 @@ struct nuc34_ctx {
   uint64_t WA = 0, WC = 0;           // words of the Aout/Ain rows and of the Cm rows
   DevBuf offA, triA, triBC;
   DevBuf cSlot;                      // per c-slot: triangle id, in-list position of its a-edge
+  DevBuf vbins;                      // degree-binned schedule: vertex classes (during the triangle passes)
   // support
   bool has_sup = false;
   uint64_t Q = 0;
@@ -318,7 +319,7 @@ void reset_stages(nuc34_ctx* c, int from) {
   }
   if (from <= 1) {
     for (DevBuf* b : {&c->IO, &c->IE, &c->IEe, &c->inpos, &c->rowA, &c->rowC, &c->Aout, &c->Ain, &c->Cm, &c->offA,
-                      &c->triA, &c->triBC, &c->cSlot})
+                      &c->triA, &c->triBC, &c->cSlot, &c->vbins})
       b->release();
     c->has_tris = false;
     c->T = 0;
@@ -354,6 +355,8 @@ std::vector<std::pair<uint64_t, uint64_t>> edge_parts(nuc34_ctx* c, uint32_t par
 }
 
 uint32_t tri_tile(const nuc34_ctx* c) { return c->tune.tri_tile_edges ? c->tune.tri_tile_edges : 64u; }
+// Degree-binned intersection: owners up to this out-degree stay on the edge tiles.
+uint32_t tri_bins_min_degree(const nuc34_ctx* c) { return c->tune.tri_schedule == 2 ? 8u : 0u; }
 
 BitRows bitrows(nuc34_ctx* c) {
   return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<RankWord>(),
@@ -400,11 +403,22 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     zero(c, c->Aout);
     zero(c, c->Ain);
     zero(c, c->Cm);
+    const uint32_t min_deg = tri_bins_min_degree(c);
+    if (c->tune.tri_schedule && n) {
+      // degree-binned schedule: the vertices of each class, for both passes
+      c->vbins = c->alloc.get(3 * static_cast<uint64_t>(n) * 4 + 16, "vertex classes");
+      uint32_t* const counts = c->vbins.as<uint32_t>() + 3 * static_cast<uint64_t>(n);
+      NUC34_HIP(hipMemsetAsync(counts, 0, 16, c->s));
+      launch_vertex_bins(c->O.as<uint32_t>(), n, min_deg, c->vbins.as<uint32_t>(), counts, c->s);
+    }
+    const VertexBins bins{c->vbins.as<uint32_t>(), c->vbins.as<uint32_t>() + 3 * static_cast<uint64_t>(n), n,
+                          min_deg};
+    const VertexBins* pb = c->vbins ? &bins : nullptr;
     const BitRows rows = bitrows(c);
     ktime(c, K_TRI_BITS, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_bits(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
-                        c->inpos.as<uint32_t>(), tri_tile(c), c->s);
+                        c->inpos.as<uint32_t>(), tri_tile(c), pb, c->s);
     });
   });
   // canonical order: role counts, ids and ranks, then the records at their ids
@@ -432,12 +446,15 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
     c->cSlot = c->alloc.get(c->T * 8 + 8, "triangles by c-slot");
+    const VertexBins bins{c->vbins.as<uint32_t>(), c->vbins.as<uint32_t>() + 3 * static_cast<uint64_t>(n), n,
+                          tri_bins_min_degree(c)};
     ktime(c, K_TRI_EMIT, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
                         c->inpos.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                        c->cSlot.as<uint2>(), tri_tile(c), c->s);
+                        c->cSlot.as<uint2>(), tri_tile(c), c->vbins ? &bins : nullptr, c->s);
     });
+    c->vbins.release();
   });
   c->st.ms_triangles = ms_enum + c->st.ms_tri_order;
   c->has_tris = true;
@@ -1349,6 +1366,7 @@ int nuc34_set_tuning(nuc34_ctx* c, const nuc34_tuning* t) {
   if (t->tri_tile_edges && t->tri_tile_edges != 16 && t->tri_tile_edges != 32 && t->tri_tile_edges != 64)
     return fail(c, NUC34_E_INVALID, "tri_tile_edges must be 16, 32 or 64");
   if (t->batch_peel > 2) return fail(c, NUC34_E_INVALID, "batch_peel must be 0, 1 or 2");
+  if (t->tri_schedule > 2) return fail(c, NUC34_E_INVALID, "tri_schedule must be 0, 1 or 2");
   c->tune = *t;
   c->tune.peel_rounds_per_launch &= ~1u;
   return NUC34_OK;
@@ -1407,6 +1425,7 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
       {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, false, false},
       {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false, false},
       {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false, false},
+      {&nuc34_tuning::tri_schedule, {0, 1, 2}, false, false, false},
       {&nuc34_tuning::tri_tile_edges, {16, 32, 64}, false, false, false},
       {&nuc34_tuning::peel_finish_grid, {256, 512, 1024}, false, false, false},
   };

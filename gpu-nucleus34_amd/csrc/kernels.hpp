@@ -20,16 +20,30 @@ void launch_check_csr(const uint32_t* O, const uint32_t* E, uint32_t n, uint64_t
 void launch_batch_union(const uint32_t* O, const uint32_t* E, const uint64_t* vfirst, const uint64_t* efirst,
                         uint32_t count, uint64_t nv, uint32_t* Ou, uint32_t* Eu, uint32_t* bad, hipStream_t s);
 
-// (b) triangle enumeration over oriented edges [e_begin, e_end): the bits pass
-// fills the incidence rows, the emit pass (after the row ranks) writes the
-// triangles at their canonical indices and, at their c-slots, their ids with
-// the in-list position of their a-edge (cSlot). `tile` = oriented edges per wave
-// tile: 16, 32 or 64 (std::logic_error otherwise).
+// (b) triangle enumeration over oriented edges [e_begin, e_end) (whole
+// vertices): the bits pass fills the incidence rows, the emit pass (after the
+// row ranks) writes the triangles at their canonical indices and, at their
+// c-slots, their ids with the in-list position of their a-edge (cSlot).
+// `tile` = oriented edges per wave tile: 16, 32 or 64 (std::logic_error
+// otherwise). With `bins` (the degree-binned schedule), owners of out-degree
+// min_deg+1..640 are enumerated a vertex at a time by a wave or a workgroup
+// and the tiles take the rest.
+struct VertexBins {
+  const uint32_t* lists;   // 3 * n entries: vertices of each class
+  const uint32_t* counts;  // 3 counts (on the device)
+  uint32_t n;
+  uint32_t min_deg;        // owners of out-degree up to this stay on the tiles
+};
+// Sorts the vertices of out-degree min_deg+1..640 into the three classes
+// (counts zeroed by the caller); min_deg < 64.
+void launch_vertex_bins(const uint32_t* O, uint32_t n, uint32_t min_deg, uint32_t* lists, uint32_t* counts,
+                        hipStream_t s);
 void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint32_t* inpos, uint32_t tile, hipStream_t s);
+                     const BitRows& rows, const uint32_t* inpos, uint32_t tile, const VertexBins* bins,
+                     hipStream_t s);
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
                      const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint2* cSlot,
-                     uint32_t tile, hipStream_t s);
+                     uint32_t tile, const VertexBins* bins, hipStream_t s);
 
 // abc[3i..3i+2] = (a, b, c) of triangles [0, count) of the given arrays.
 void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s);

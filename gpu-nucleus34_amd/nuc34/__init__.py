@@ -62,7 +62,8 @@ class Tuning(C.Structure):
     _fields_ = [("peel_debit_grid", C.c_uint32), ("peel_tile_cap", C.c_uint32),
                 ("peel_rounds_per_launch", C.c_uint32), ("k4_list_grid_cap", C.c_uint32),
                 ("tri_tile_edges", C.c_uint32), ("peel_finish_grid", C.c_uint32),
-                ("batch_peel", C.c_uint32), ("graph_peel_max_tris", C.c_uint32)]
+                ("batch_peel", C.c_uint32), ("graph_peel_max_tris", C.c_uint32),
+                ("tri_schedule", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -207,6 +207,11 @@ typedef struct nuc34_tuning {
                                       workgroup per graph, the whole peel one launch, when no graph has
                                       more than graph_peel_max_tris triangles; 0 = 2 (0) */
   uint32_t graph_peel_max_tris;    /* most triangles of a graph peeled by one workgroup (1 << 18) */
+  uint32_t tri_schedule;           /* triangle intersection: 0 = wave tiles of tri_tile_edges oriented edges;
+                                      1 = by the owner's out-degree d: a wave per vertex (d <= 64), a
+                                      256-thread workgroup (d <= 256) or a 1024-thread one (d <= 640), with
+                                      the vertex's rows built in LDS, and the tiles above 640; 2 = as 1 with
+                                      the tiles also for d <= 8 (0) */
 } nuc34_tuning;
 
 int nuc34_set_tuning(nuc34_ctx* ctx, const nuc34_tuning* tuning);

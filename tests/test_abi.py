@@ -49,7 +49,8 @@ def test_struct_layouts_match_header(tmp_path):
               "nuc34_stats": (nuc34.Stats, ["k_max", "ms_total", "n_kernels", "kernels", "debits", "ms_tri_order",
                                              "ms_peel_init"]),
               "nuc34_tuning": (nuc34.Tuning, ["peel_tile_cap", "k4_list_grid_cap", "tri_tile_edges",
-                                                "peel_finish_grid", "batch_peel", "graph_peel_max_tris"]),
+                                                "peel_finish_grid", "batch_peel", "graph_peel_max_tris",
+                                                "tri_schedule"]),
               "nuc34_device_rates": (nuc34.DeviceRates, ["atomic_gops", "load_hbm_gops"])}
     src = ["#include <stdio.h>", "#include <stddef.h>", '#include "nuc34.h"', "int main(void) {"]
     for st, (_, fs) in fields.items():

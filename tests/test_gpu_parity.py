@@ -481,6 +481,55 @@ def test_malformed_csr_rejected(case):
         assert st["triangles"] == 2 and st["four_cliques"] == 0
 
 
+def _hub_core_graph(seed=7):
+    """900 hubs with 870 private leaves each and about 40 random hub neighbours;
+    X sees all 900 hubs (out-degree 900 > 640: edge tiles), Y the first 400
+    (a 1024-thread workgroup), Z the first 200 (a 256-thread workgroup), X-Y is
+    an edge; hubs (about 20 out-neighbours) and leaves take a wave each."""
+    rng = np.random.default_rng(seed)
+    H, L = 900, 870
+    X, Y, Z = H, H + 1, H + 2
+    src, dst = [], []
+    for h in range(H):
+        leaves = np.arange(H + 3 + h * L, H + 3 + (h + 1) * L)
+        src += [h] * L
+        dst += leaves.tolist()
+    a = rng.integers(0, H, 18000)
+    b = rng.integers(0, H, 18000)
+    src += a.tolist()
+    dst += b.tolist()
+    src += [X] * H + [Y] * 400 + [Z] * 200 + [X]
+    dst += list(range(H)) + list(range(400)) + list(range(200)) + [Y]
+    return nuc34.graph_from_pairs(np.array(src, dtype=np.uint32), np.array(dst, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("variant,parts", [("direct", 1), ("ondemand", 1), ("direct", 3), ("ondemand", 2)])
+def test_degree_binned_intersection_matches_tiles(variant, parts):
+    """The degree-binned intersection (tri_schedule 1: a wave, a 256-thread or
+    a 1024-thread workgroup per vertex by out-degree, edge tiles above 640; 2:
+    also tiles up to out-degree 8) gives the edge tiles' triangle list,
+    support, k and rounds, on R-MAT s16
+    (owners up to out-degree 248) and on a hub-core graph with owners in every
+    class; the hub-core results also equal the oracle's."""
+    hub = _hub_core_graph()
+    ref = oracle34.OracleGraph.from_csr(hub.O, hub.E, orient=True).run("ondemand", 4)
+    for name, g in (("hub-core", hub), ("rmat16", nuc34.gen_preset("rmat16"))):
+        out = []
+        for schedule in (0, 1, 2):
+            print(f"[binned] {name} {variant} parts={parts} tri_schedule={schedule}", flush=True)
+            with nuc34.Context() as ctx:
+                ctx.set_tuning(tri_schedule=schedule)
+                ctx.load_graph(g)
+                st = ctx.decompose(variant, parts)
+                out.append((ctx.copy_triangles(), ctx.copy_support(), ctx.copy_k(), st))
+        t0, s0, k0, st0 = out[0]
+        for t1, s1, k1, st1 in out[1:]:
+            assert np.array_equal(t0, t1) and np.array_equal(s0, s1) and np.array_equal(k0, k1), name
+            assert st0["rounds"] == st1["rounds"] and st0["four_cliques"] == st1["four_cliques"], name
+        if name == "hub-core":
+            assert np.array_equal(t1, ref["tris"]) and np.array_equal(k1, ref["k"]) and st1["rounds"] == ref["rounds"]
+
+
 def test_autotune_and_launch_shapes_keep_results(tmp_path):
     """Launch shapes change speed, never results: the autotuned shape and odd
     explicit shapes give the same k and round count in both variants."""
@@ -496,7 +545,8 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
         for variant, shape in (("ondemand", {}), ("ondemand", dict(peel_debit_grid=96, peel_tile_cap=3,
                                                                     peel_rounds_per_launch=6)),
                                ("direct", dict(peel_debit_grid=300, k4_list_grid_cap=17)),
-                               ("direct", dict(tri_tile_edges=16)), ("ondemand", dict(tri_tile_edges=32))):
+                               ("direct", dict(tri_tile_edges=16)), ("ondemand", dict(tri_tile_edges=32)),
+                               ("ondemand", dict(tri_schedule=1)), ("direct", dict(tri_schedule=0))):
             if shape:
                 ctx.set_tuning(**shape)
             ctx.reset()
@@ -506,6 +556,8 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
             ctx.set_tuning(peel_tile_cap=65)
         with pytest.raises(nuc34.Nuc34Error):
             ctx.set_tuning(tri_tile_edges=48)
+        with pytest.raises(nuc34.Nuc34Error):
+            ctx.set_tuning(tri_schedule=3)
 
 
 def level_peel_sim(tris, O, E):
