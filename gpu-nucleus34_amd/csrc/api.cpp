@@ -513,6 +513,31 @@ PeelShape peel_shape(const nuc34_ctx* c) {
   return sh;
 }
 
+// The graphs' triangle slices of a loaded batch (count + 1 offsets, from offA at
+// the graphs' first edges).
+std::vector<uint64_t> batch_tri_first(nuc34_ctx* c) {
+  const uint64_t k = c->batch_e.size();
+  std::vector<uint64_t> tf(k);
+  DevBuf idx = c->alloc.get(k * 8, "batch boundaries");
+  DevBuf out = c->alloc.get(k * 8, "batch triangle offsets");
+  NUC34_HIP(hipMemcpyAsync(idx.as<void>(), c->batch_e.data(), k * 8, hipMemcpyHostToDevice, c->s));
+  launch_gather_u64(c->offA.as<uint64_t>(), idx.as<uint64_t>(), out.as<uint64_t>(), k, c->s);
+  NUC34_HIP(hipMemcpyAsync(tf.data(), out.as<void>(), k * 8, hipMemcpyDeviceToHost, c->s));
+  NUC34_HIP(hipStreamSynchronize(c->s));
+  if (tf.front() != 0 || tf.back() != c->T) throw std::logic_error("batch triangle offsets inconsistent");
+  for (uint64_t g = 0; g + 1 < k; ++g)
+    if (tf[g + 1] < tf[g]) throw std::logic_error("batch triangle offsets not monotone");
+  return tf;
+}
+
+// Every graph of the batch small enough for one workgroup's peel.
+bool graphs_fit_one_workgroup(const nuc34_ctx* c, const std::vector<uint64_t>& tf) {
+  const uint64_t most = c->tune.graph_peel_max_tris ? c->tune.graph_peel_max_tris : kGraphPeelMaxTris;
+  for (size_t g = 0; g + 1 < tf.size(); ++g)
+    if (tf[g + 1] - tf[g] > most) return false;
+  return true;
+}
+
 void run_peel(nuc34_ctx* c, int variant) {
   const uint64_t T = c->T;
   c->kv = c->alloc.get(T * 4 + 4, "k values");
@@ -552,25 +577,14 @@ void run_peel(nuc34_ctx* c, int variant) {
   std::vector<uint64_t> tfirst;
   DevBuf gtf, gord, gout;
   GraphQueue gq{};
-  const uint32_t graph_max = c->tune.graph_peel_max_tris ? c->tune.graph_peel_max_tris : kGraphPeelMaxTris;
   bool by_graph = ngraphs > 0 && c->tune.batch_peel != 1 && !c->noalloc;
   if (by_graph) {
-    gtf = c->alloc.get((ngraphs + 1) * 8, "batch triangle offsets");
-    {
-      DevBuf idx = c->alloc.get((ngraphs + 1) * 8, "batch boundaries");
-      NUC34_HIP(hipMemcpyAsync(idx.as<void>(), c->batch_e.data(), (ngraphs + 1) * 8, hipMemcpyHostToDevice, c->s));
-      launch_gather_u64(c->offA.as<uint64_t>(), idx.as<uint64_t>(), gtf.as<uint64_t>(), ngraphs + 1, c->s);
-      tfirst.resize(ngraphs + 1);
-      NUC34_HIP(hipMemcpyAsync(tfirst.data(), gtf.as<void>(), (ngraphs + 1) * 8, hipMemcpyDeviceToHost, c->s));
-      NUC34_HIP(hipStreamSynchronize(c->s));
-    }
-    if (tfirst.front() != 0 || tfirst.back() != T) throw std::logic_error("batch triangle offsets inconsistent");
-    for (uint64_t g = 0; g < ngraphs && by_graph; ++g) {
-      if (tfirst[g + 1] < tfirst[g]) throw std::logic_error("batch triangle offsets not monotone");
-      by_graph = tfirst[g + 1] - tfirst[g] <= graph_max;
-    }
+    tfirst = batch_tri_first(c);
+    by_graph = graphs_fit_one_workgroup(c, tfirst);
   }
   if (by_graph) {
+    gtf = c->alloc.get((ngraphs + 1) * 8, "batch triangle offsets");
+    NUC34_HIP(hipMemcpyAsync(gtf.as<void>(), tfirst.data(), (ngraphs + 1) * 8, hipMemcpyHostToDevice, c->s));
     std::vector<uint32_t> order(ngraphs);
     for (uint32_t g = 0; g < ngraphs; ++g) order[g] = g;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
@@ -1264,11 +1278,15 @@ int nuc34_decompose(nuc34_ctx* c, int variant, uint32_t parts, nuc34_stats* stat
     // K4s from the ranked rows at O(1) per K4 and skips K4s destroyed in
     // earlier rounds, so it needs no 48 bytes per K4 and ran faster than the
     // direct variant on every measured single graph (10^4 to 10^10 K4s). AUTO
-    // for a batch of graphs is AUTO_FIT: the union's graphs are small, their
-    // lists cheap to build, and the direct peel one workgroup per graph beat
-    // the on-demand one (512-graph mix: 22.6 against 26.1 ms).
+    // for a batch of small graphs (each within one workgroup's peel) is
+    // AUTO_FIT: their lists are cheap to build, and the direct peel beat the
+    // on-demand one (512-graph mix: 22.6 against 26.1 ms). A batch of large
+    // graphs stays on demand (two R-MAT s18: 1174 ms direct, against 362 ms
+    // per graph on demand for four).
     int v = variant == NUC34_AUTO ? NUC34_ONDEMAND : variant;
-    if (variant == NUC34_AUTO_FIT || (variant == NUC34_AUTO && c->batch_e.size() > 2)) {
+    const bool small_batch =
+        variant == NUC34_AUTO && c->batch_e.size() > 2 && graphs_fit_one_workgroup(c, batch_tri_first(c));
+    if (variant == NUC34_AUTO_FIT || small_batch) {
       // the direct variant's extra memory: the lists (48 bytes per K4) and
       // their offsets, plus the peel's O(T) state both variants hold
       const uint64_t need = 48 * c->Q + 8 * (c->T + 1) + 20 * c->T;

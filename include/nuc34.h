@@ -178,7 +178,8 @@ int nuc34_copy_support(nuc34_ctx* ctx, int32_t* support);
  * K4 lists in HBM, 48 bytes per K4), NUC34_ONDEMAND (K4s re-enumerated from the
  * bit-packed incidence as triangles die, O(T) memory), NUC34_AUTO (one graph:
  * on demand, never slower and never larger on the measured single graphs; a
- * batch of two or more graphs: as NUC34_AUTO_FIT) or NUC34_AUTO_FIT
+ * batch of two or more graphs each within tuning graph_peel_max_tris
+ * triangles: as NUC34_AUTO_FIT) or NUC34_AUTO_FIT
  * (direct when, with Q known from the support count, its lists fit what is
  * left of the context's budget, or of free device memory without one, with a
  * quarter to spare; on demand otherwise: the reference's own choice between
@@ -285,8 +286,8 @@ int nuc34_decompose_noalloc(int device, const uint32_t* O, const uint32_t* E, ui
  * large-job lane runs graphs at or above `large_edges` edge entries or
  * `large_wedges` wedges one at a time, so they never hold small jobs back.
  * Admission is by HBM footprint: each lane holds its share of the budget, a batch runs
- * in the configured variant (NUC34_AUTO: on demand for a batch of one graph,
- * direct when its lists fit for a batch of several), and a batch
+ * in the configured variant (NUC34_AUTO: direct when its lists fit for a batch
+ * of several small graphs, on demand otherwise), and a batch
  * that runs out of its lane's budget is split in halves and retried.
  */
 typedef struct nuc34_server nuc34_server;

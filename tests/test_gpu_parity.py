@@ -366,6 +366,24 @@ def test_batch_peel_one_workgroup_per_graph(variant):
     assert rounds[7] == 1 and (k2[int(tf[7]):int(tf[8])] == 9).all()  # gs[7] is K_12
 
 
+def test_auto_variant_for_batches():
+    """AUTO runs a batch of small graphs direct (their lists fit) and a batch
+    holding a graph above graph_peel_max_tris on demand; one graph on demand."""
+    gs = [nuc34.gen_batch_graph(3, i) for i in range(4)]
+    with nuc34.Context() as ctx:
+        ctx.load_batch(gs)
+        assert ctx.decompose("auto")["variant"] == nuc34.DIRECT
+        k_small = ctx.copy_k()
+        ctx.triangles()
+        most = max(ctx.batch_graph(g)["tri_count"] for g in range(len(gs)))
+        ctx.set_tuning(graph_peel_max_tris=max(most - 1, 1))
+        ctx.reset()
+        assert ctx.decompose("auto")["variant"] == nuc34.ONDEMAND
+        assert np.array_equal(ctx.copy_k(), k_small)
+        ctx.load_graph(gs[0])
+        assert ctx.decompose("auto")["variant"] == nuc34.ONDEMAND
+
+
 def test_batch_peel_falls_back_to_shared_rounds_for_large_graphs():
     """A graph above graph_peel_max_tris keeps the batch on shared rounds (no
     per-graph rounds), with the same k; raising the limit peels it by graph."""
