@@ -370,10 +370,15 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
 // ---- support, K4 lists, canonical K4 list -------------------------------------
 
 // Support and owned counts, a wave per 64 triangles: the triangles' row words
-// (W up words of a and b, then WC down words) are flattened over the lanes 64
-// at a time, so a triangle whose lowest vertex has thousands of in-neighbours
-// (thousands of down words) is spread over many lanes instead of one; the
-// per-triangle sums gather in LDS.
+// (W up words of a and b, then WC down words) are flattened over the lanes in
+// chunks of kSupChunk consecutive words, 64 chunks at a time, so a triangle
+// whose lowest vertex has thousands of in-neighbours (thousands of down words)
+// is spread over many lanes instead of one, and each lane's slot search and
+// shuffles serve several words; the per-triangle sums gather in LDS. Chunks of
+// 2 measured best on R-MAT s20 + 64 K30 (53.6 ms; 1, 4 and 8 words: 82, 71 and
+// 71 ms): longer chunks spread a load instruction's lanes over more lines.
+constexpr uint32_t kSupChunk = 2;
+
 __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __restrict__ sup,
                                                          uint32_t* __restrict__ own, uint64_t* __restrict__ total) {
   __shared__ uint32_t s_pre[kWPB][kWave + 1];
@@ -398,30 +403,55 @@ __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __res
       cb = s.rows.rowC[b];
       WC = static_cast<uint32_t>(s.rows.rowC[a + 1] - ca);  // a and b are out-edges of one vertex
     }
-    const uint32_t words = W + WC;
-    const uint32_t inc = wave_inclusive_sum(words);
-    const uint32_t nwords = __shfl(inc, kWave - 1);
-    s_pre[w][lane] = inc - words;
+    // chunks: the up words and the down words are chunked separately
+    const uint32_t cu = (W + kSupChunk - 1) / kSupChunk;
+    const uint32_t chunks = cu + (WC + kSupChunk - 1) / kSupChunk;
+    const uint32_t inc = wave_inclusive_sum(chunks);
+    const uint32_t nchunks = __shfl(inc, kWave - 1);
+    s_pre[w][lane] = inc - chunks;
     if (lane == kWave - 1) s_pre[w][kWave] = inc;
     s_sup[w][lane] = 0;
     s_own[w][lane] = 0;
     wave_sync();
-    for (uint32_t base = 0; base < nwords; base += kWave) {
+    for (uint32_t base = 0; base < nchunks; base += kWave) {
       const uint32_t item = base + lane;
-      const bool act = item < nwords;
-      const uint32_t i = find_slot(s_pre[w], act ? item : nwords - 1);
+      const bool act = item < nchunks;
+      const uint32_t i = find_slot(s_pre[w], act ? item : nchunks - 1);
       // triangle i's rows, from lane i (every lane takes part)
       const uint64_t rai = shfl_u64(ra, i), rbi = shfl_u64(rb, i), cai = shfl_u64(ca, i), cbi = shfl_u64(cb, i);
-      const uint32_t Wi = shfl_u32(W, i);
+      const uint32_t Wi = shfl_u32(W, i), WCi = shfl_u32(WC, i);
       if (act) {
-        const uint32_t wi = item - s_pre[w][i];
+        const uint32_t ci = item - s_pre[w][i];
+        const uint32_t cui = (Wi + kSupChunk - 1) / kSupChunk;
         uint32_t c = 0, o = 0;
-        if (wi < Wi) {
-          const uint32_t oa = s.rows.Aout[rai + wi], ob = s.rows.Aout[rbi + wi];
-          c = static_cast<uint32_t>(__popc((oa | s.rows.Ain[rai + wi].bits) & (ob | s.rows.Ain[rbi + wi].bits)));
-          o = static_cast<uint32_t>(__popc(oa & ob));
+        // every load of the chunk is issued before any is used
+        if (ci < cui) {
+          const uint32_t w0 = ci * kSupChunk;
+          uint32_t oa[kSupChunk], ob[kSupChunk], ia[kSupChunk], ib[kSupChunk];
+#pragma unroll
+          for (uint32_t j = 0; j < kSupChunk; ++j) {
+            const bool in = w0 + j < Wi;
+            oa[j] = in ? s.rows.Aout[rai + w0 + j] : 0u;
+            ob[j] = in ? s.rows.Aout[rbi + w0 + j] : 0u;
+            ia[j] = in ? s.rows.Ain[rai + w0 + j].bits : 0u;
+            ib[j] = in ? s.rows.Ain[rbi + w0 + j].bits : 0u;
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kSupChunk; ++j) {
+            c += static_cast<uint32_t>(__popc((oa[j] | ia[j]) & (ob[j] | ib[j])));
+            o += static_cast<uint32_t>(__popc(oa[j] & ob[j]));
+          }
         } else {
-          c = static_cast<uint32_t>(__popc(s.rows.Cm[cai + (wi - Wi)].bits & s.rows.Cm[cbi + (wi - Wi)].bits));
+          const uint32_t w0 = (ci - cui) * kSupChunk;
+          uint32_t xa[kSupChunk], xb[kSupChunk];
+#pragma unroll
+          for (uint32_t j = 0; j < kSupChunk; ++j) {
+            const bool in = w0 + j < WCi;
+            xa[j] = in ? s.rows.Cm[cai + w0 + j].bits : 0u;
+            xb[j] = in ? s.rows.Cm[cbi + w0 + j].bits : 0u;
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kSupChunk; ++j) c += static_cast<uint32_t>(__popc(xa[j] & xb[j]));
         }
         if (c) atomicAdd(&s_sup[w][i], c);
         if (o) atomicAdd(&s_own[w][i], o);
