@@ -426,22 +426,24 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
     const BitRows rows = bitrows(c);
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
-      // a-role counts give each edge's first triangle id, c-role counts its
-      // first c-slot; both become the running ranks of the row words
-      DevBuf cntA = c->alloc.get(m * 4 + 4, "a-role counts");
-      DevBuf cntC = c->alloc.get(m * 4 + 4, "c-role counts");
-      DevBuf cOff = c->alloc.get((m + 1) * 8, "c-slot offsets");
-      launch_row_counts(rows, m, cntA.as<uint32_t>(), cntC.as<uint32_t>(), c->s);
-      DevBuf scratch = c->alloc.get(scan_scratch_elems(m) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(cntA.as<uint32_t>(), c->offA.as<uint64_t>(), m, scratch.as<uint64_t>(),
-                                         c->s);
-      exclusive_scan<uint32_t, uint64_t>(cntC.as<uint32_t>(), cOff.as<uint64_t>(), m, scratch.as<uint64_t>(), c->s);
-      c->T = read_scalar<uint64_t>(c, c->offA.as<uint64_t>() + m);
+      // one exclusive scan of the set bits over all a-role words gives every
+      // word its running rank (the triangle id of its first bit), and at each
+      // row's first word the edge's first triangle id; likewise the c-role
+      // words and the c-slots
+      const uint64_t WA = c->WA, WC = c->WC;
+      DevBuf pA = c->alloc.get((WA + 1) * 8, "a-role ranks");
+      DevBuf pC = c->alloc.get((WC + 1) * 8, "c-role ranks");
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(std::max(WA, WC)) * 8, "scan scratch");
+      exclusive_scan<RankWord, uint64_t>(c->Ain.as<RankWord>(), pA.as<uint64_t>(), WA, scratch.as<uint64_t>(), c->s);
+      exclusive_scan<RankWord, uint64_t>(c->Cm.as<RankWord>(), pC.as<uint64_t>(), WC, scratch.as<uint64_t>(), c->s);
+      c->T = read_scalar<uint64_t>(c, pA.as<uint64_t>() + WA);
       if (c->T >= 0xFFFFFFFFull)
         throw Overflow("triangle count " + std::to_string(c->T) + " exceeds the 32-bit triangle id range");
-      if (read_scalar<uint64_t>(c, cOff.as<uint64_t>() + m) != c->T)
+      if (read_scalar<uint64_t>(c, pC.as<uint64_t>() + WC) != c->T)
         throw std::logic_error("c-role bits disagree with the triangle count");
-      launch_row_ranks(rows, m, c->offA.as<uint64_t>(), cOff.as<uint64_t>(), c->s);
+      launch_set_ranks(c->Ain.as<RankWord>(), pA.as<uint64_t>(), WA, c->s);
+      launch_set_ranks(c->Cm.as<RankWord>(), pC.as<uint64_t>(), WC, c->s);
+      launch_gather_u64(pA.as<uint64_t>(), c->rowA.as<uint64_t>(), c->offA.as<uint64_t>(), m + 1, c->s);
     }
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");

@@ -48,17 +48,6 @@ __global__ void k_row_words(const uint32_t* __restrict__ O, const uint32_t* __re
   }
 }
 
-__global__ void k_row_counts(BitRows r, uint64_t m, uint32_t* __restrict__ cntA, uint32_t* __restrict__ cntC) {
-  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t a = 0, c = 0;
-    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i) a += static_cast<uint32_t>(__popc(r.Ain[i].bits));
-    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) c += static_cast<uint32_t>(__popc(r.Cm[i].bits));
-    cntA[e] = a;
-    cntC[e] = c;
-  }
-}
-
 __global__ void k_edge_triangles(BitRows r, uint64_t m, uint32_t* __restrict__ cnt) {
   for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
        e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -70,24 +59,13 @@ __global__ void k_edge_triangles(BitRows r, uint64_t m, uint32_t* __restrict__ c
   }
 }
 
-// Running ranks: every Ain word of e gets offA[e] plus the set bits of e's row
-// before it, every Cm word cOff[e] plus the bits before it. Ranks are triangle
-// ids and c-slots, which stay below 2^32 (checked by the caller).
-__global__ void k_row_ranks(BitRows r, uint64_t m, const uint64_t* __restrict__ offA,
-                            const uint64_t* __restrict__ cOff) {
-  for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-       e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t run = static_cast<uint32_t>(offA[e]);
-    for (uint64_t i = r.rowA[e]; i < r.rowA[e + 1]; ++i) {
-      r.Ain[i].rank = run;
-      run += static_cast<uint32_t>(__popc(r.Ain[i].bits));
-    }
-    run = static_cast<uint32_t>(cOff[e]);
-    for (uint64_t i = r.rowC[e]; i < r.rowC[e + 1]; ++i) {
-      r.Cm[i].rank = run;
-      run += static_cast<uint32_t>(__popc(r.Cm[i].bits));
-    }
-  }
+// rank[i] = P[i]: the exclusive scan of the words' set bits over the whole
+// array (rows are laid out edge after edge, so the scan at a row's first word
+// is that edge's first triangle id, or first c-slot).
+__global__ void k_set_ranks(RankWord* __restrict__ w, const uint64_t* __restrict__ P, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    w[i].rank = static_cast<uint32_t>(P[i]);
 }
 
 }  // namespace
@@ -112,12 +90,10 @@ void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uin
                      indeg.as<uint32_t>(), keys.as<uint64_t>());
   NUC34_LAUNCH_CHECK();
   {
-    // segment offsets as u64 for the segmented sort
+    // segment offsets as u64 for the segmented sort; k_in_fill's per-vertex
+    // cursors have counted every in-edge again, so they are the in-degrees
     DevBuf off64 = alloc.get((static_cast<uint64_t>(n) + 1) * 8, "in-list offsets");
     DevBuf scratch = alloc.get(scan_scratch_elems(n) * 8, "scan scratch");
-    NUC34_HIP(hipMemsetAsync(indeg.as<void>(), 0, indeg.bytes(), s));
-    hipLaunchKernelGGL(k_indeg, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, E, m, indeg.as<uint32_t>());
-    NUC34_LAUNCH_CHECK();
     exclusive_scan<uint32_t, uint64_t>(indeg.as<uint32_t>(), off64.as<uint64_t>(), n, scratch.as<uint64_t>(), s);
     segmented_sort_u64(keys.as<uint64_t>(), off64.as<uint64_t>(), n, alloc, s);
   }
@@ -133,21 +109,15 @@ void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s) {
-  if (!m) return;
-  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cntA, cntC);
-  NUC34_LAUNCH_CHECK();
-}
-
 void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStream_t s) {
   if (!m) return;
   hipLaunchKernelGGL(k_edge_triangles, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cnt);
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_row_ranks(const BitRows& r, uint64_t m, const uint64_t* offA, const uint64_t* cOff, hipStream_t s) {
-  if (!m) return;
-  hipLaunchKernelGGL(k_row_ranks, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, offA, cOff);
+void launch_set_ranks(RankWord* w, const uint64_t* P, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_set_ranks, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, w, P, n);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -62,11 +62,11 @@ void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uin
 // Per-edge row lengths in words: wa[e] = ceil(d+(src e)/32), wc[e] = ceil(d-(src e)/32).
 void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src, uint64_t m, uint32_t* wa,
                       uint32_t* wc, hipStream_t s);
-// Role counts from the rows: cntA = |Ain|, cntC = |Cm| per edge.
-void launch_row_counts(const BitRows& r, uint64_t m, uint32_t* cntA, uint32_t* cntC, hipStream_t s);
 // Triangles on each edge, in any role: cnt[e] = |Aout| + |Ain| + |Cm| of e.
 void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStream_t s);
-// Running ranks of the Ain words (from offA) and Cm words (from cOff).
-void launch_row_ranks(const BitRows& r, uint64_t m, const uint64_t* offA, const uint64_t* cOff, hipStream_t s);
+// Running ranks: w[i].rank = P[i], P the exclusive scan of the words' set bits
+// (exclusive_scan over RankWord). Rows lie edge after edge, so P at a row's
+// first word is the edge's first triangle id (Ain) or first c-slot (Cm).
+void launch_set_ranks(RankWord* w, const uint64_t* P, uint64_t n, hipStream_t s);
 
 }  // namespace nuc34

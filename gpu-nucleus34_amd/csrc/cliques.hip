@@ -621,26 +621,27 @@ __global__ void k_peel_ctl_init(PeelCtl* ctl) {
 }
 
 // dd = support, kv = -1, every triangle alive (padding fields kGone), and the
-// first level's minimum into ctl[0].
+// first level's minimum into ctl[0]. Consecutive threads take consecutive
+// triangles (and consecutive state words), so every access is coalesced.
 __global__ __launch_bounds__(kBlock) void k_peel_init(PeelBufs p, const int32_t* __restrict__ sup) {
   const uint64_t W = (p.T + 15) / 16;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   uint32_t mn = kNoMin;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < W;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t w = 0;
-#pragma unroll
-    for (uint32_t f = 0; f < 16; ++f) {
-      const uint64_t t = 16 * i + f;
-      if (t < p.T) {
-        const int32_t d = sup[t];
-        p.dd[t] = d;
-        p.kv[t] = -1;
-        mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
-      } else {
-        p.dd[t] = 0;
-        w |= kGone << (2 * f);
-      }
+  for (uint64_t t = gtid; t < 16 * W; t += stride) {
+    if (t < p.T) {
+      const int32_t d = sup[t];
+      p.dd[t] = d;
+      p.kv[t] = -1;
+      mn = min(mn, static_cast<uint32_t>(d < 0 ? 0 : d));
+    } else {
+      p.dd[t] = 0;  // padding of the last state word
     }
+  }
+  for (uint64_t i = gtid; i < W; i += stride) {
+    uint32_t w = 0;  // every field kAlive (0) but the padding past T
+    for (uint32_t f = 0; f < 16; ++f)
+      if (16 * i + f >= p.T) w |= kGone << (2 * f);
     p.st[i] = w;
   }
   block_min_to(mn, &p.ctl[0].minval);
@@ -1305,7 +1306,7 @@ void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t
 
 void launch_peel_init(const PeelBufs& p, const int32_t* sup, hipStream_t s) {
   hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(1), 0, s, p.ctl);
-  hipLaunchKernelGGL(k_peel_init, dim3(grid_for((p.T + 15) / 16, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
+  hipLaunchKernelGGL(k_peel_init, dim3(grid_for(p.T, kBlock, 4096)), dim3(kBlock), 0, s, p, sup);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -1,8 +1,17 @@
+#include "bitrows.hpp"
 #include "scan.hpp"
 
 namespace nuc34 {
 
 namespace {
+
+// The value a scan input contributes: itself, or for a ranked row word the
+// number of its set bits (the scan then gives every word its running rank).
+template <typename T>
+__device__ __forceinline__ T scan_value(T x) {
+  return x;
+}
+__device__ __forceinline__ uint32_t scan_value(RankWord w) { return static_cast<uint32_t>(__popc(w.bits)); }
 
 // Block-wide inclusive scan of one value per thread; returns this thread's
 // inclusive prefix and writes the block total to *total.
@@ -38,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const In* __restrict__ i
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
-    if (i < n) acc += static_cast<Out>(in[i]);
+    if (i < n) acc += static_cast<Out>(scan_value(in[i]));
   }
   Out total;
   block_inclusive<Out>(acc, lds, &total);
@@ -70,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const In* __restrict__ in
   Out carry = partial[blockIdx.x];
   for (int j = 0; j < kScanItems; ++j) {
     const uint64_t i = base + static_cast<uint64_t>(j) * kBlock + threadIdx.x;
-    const Out v = i < n ? static_cast<Out>(in[i]) : Out(0);
+    const Out v = i < n ? static_cast<Out>(scan_value(in[i])) : Out(0);
     Out total;
     const Out inc = block_inclusive<Out>(v, lds, &total);
     if (i < n) out[i] = carry + inc - v;
@@ -109,5 +118,6 @@ template void exclusive_scan<uint32_t, uint32_t>(const uint32_t*, uint32_t*, uin
 template void exclusive_scan<uint32_t, uint64_t>(const uint32_t*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template void exclusive_scan<int32_t, uint64_t>(const int32_t*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template void exclusive_scan<uint64_t, uint64_t>(const uint64_t*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
+template void exclusive_scan<RankWord, uint64_t>(const RankWord*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 
 }  // namespace nuc34

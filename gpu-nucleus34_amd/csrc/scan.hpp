@@ -1,6 +1,7 @@
 // Device-wide exclusive prefix sum (reduce-then-scan over 4096-element tiles),
 // out[0..N] with out[N] = total. Used for CSR offsets: oriented offsets,
-// triangles per edge, incidence offsets, four-cliques per triangle.
+// triangles per edge, incidence offsets, four-cliques per triangle; over
+// ranked row words (RankWord) it sums their set bits.
 #pragma once
 #include "common.hpp"
 
