@@ -1441,7 +1441,7 @@ int nuc34_autotune(nuc34_ctx* c, int variant, nuc34_tuning* best, double* best_m
   // workgroup per graph), then the launch shapes under it
   const Knob knobs[] = {
       {&nuc34_tuning::batch_peel, {1, 2}, false, false, true},
-      {&nuc34_tuning::peel_debit_grid, {1024, 2048, 4096, 8192}, false, false, false},
+      {&nuc34_tuning::peel_debit_grid, {1024, 2048, 4096, 8192, 16384}, false, false, false},
       {&nuc34_tuning::peel_tile_cap, {8, 16, 32, 64}, false, false, false},
       {&nuc34_tuning::peel_rounds_per_launch, {16, 32, 64}, false, false, false},
       {&nuc34_tuning::k4_list_grid_cap, {8192, 32768}, true, false, false},
