@@ -793,16 +793,21 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t gsize = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  // Retire the frontier and promote the pushed triangles. The on-demand peel
+  // (UNLINK) keeps both in the live and dying planes, which are all its debit
+  // reads; its level scans need only "not alive", which states 01 and 10
+  // already say, so it leaves the state words alone (two device atomics per
+  // triangle fewer). The direct debit reads the states, so it updates them.
   for (uint64_t i = gtid; i < F; i += gsize) {
     const uint32_t t = cur[i];
-    atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
     if (UNLINK) unlink_tri(s, t);
+    else atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
   }
   const uint32_t nn = in->nnext;
   for (uint64_t i = gtid; i < nn; i += gsize) {
     const uint32_t t = nxt[i];
-    atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
     if (UNLINK) mark_dying(s, t);
+    else atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
   }
   if (need || collect_round) {
     const int32_t take = collect_round ? L : L + 1;  // also the k of the collected triangles
@@ -1193,15 +1198,16 @@ __global__ __launch_bounds__(kGraphBlock) void k_peel_graphs(PeelBufs p, K4Src s
       else debit_lists(p, cur, n, w, kGraphWaves, G, lt[w], koff, lists, level, nxt, &s_nn);
       __syncthreads();
       const uint32_t nn = s_nn;
+      // as in k_peel_finish: the on-demand peel keeps its states in the planes
       for (uint32_t i = threadIdx.x; i < n; i += kGraphBlock) {
         const uint32_t t = cur[i];
-        atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
         if constexpr (LIVE) unlink_tri(s, t);
+        else atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
       }
       for (uint32_t i = threadIdx.x; i < nn; i += kGraphBlock) {
         const uint32_t t = nxt[i];
-        atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
         if constexpr (LIVE) mark_dying(s, t);
+        else atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
       }
       __syncthreads();
       uint32_t* const tmp = cur;

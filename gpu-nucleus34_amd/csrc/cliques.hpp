@@ -76,6 +76,9 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
 //   1 dies in the current round
 //   2 k fixed in the current round (pushed); dies in the next one
 //   3 died in an earlier round (also the padding fields past T)
+// The on-demand peel keeps the dying and retired triangles in the live and
+// dying planes of its rows instead; its state words only tell alive (0) from
+// not alive (any other value), which is all its level scans read.
 // The ownership test of a K4 reads only these states (4 MB per 16 M
 // triangles), so its random reads stay in L2.
 //
