@@ -563,14 +563,24 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
               b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
   K4Src v = k4src(c);
-  DevBuf liveA, liveC, icut;
+  // On demand: the peel numbers triangles in peel order (b-edge, then a-edge;
+  // the ranks of the peel rows), with triangles, c-slots, supports and k in
+  // that order, and k goes back to the canonical order at the end (pmap).
+  DevBuf liveA, liveC, icut, pmap, ptriA, ptriBC, pcslot, pkv, pbad;
   if (variant != NUC34_DIRECT) {
     liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
     liveC = c->alloc.get(c->WC * sizeof(PeelWord) + sizeof(PeelWord), "peel c rows");
     icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
+    pmap = c->alloc.get(T * 4, "peel order of the triangles");
+    ptriA = c->alloc.get(T * 4, "triangles in peel order");
+    ptriBC = c->alloc.get(T * 8, "triangles in peel order");
+    pcslot = c->alloc.get(T * 8 + 8, "c-slots in peel order");
+    pkv = c->alloc.get(T * 4 + 4, "k values in peel order");
+    pbad = c->alloc.get(8, "peel order check");
     v.PA = liveA.as<PeelWord>();
     v.PC = liveC.as<PeelWord>();
     v.icut = icut.as<uint32_t>();
+    pb.kv = pkv.as<int32_t>();
   }
   // A loaded batch whose graphs are all small: one workgroup per graph, the
   // whole peel one launch (k_peel_graphs). The graphs' triangle slices come
@@ -619,9 +629,28 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl hs[2] = {};
   PeelCtl h{};
   c->st.ms_peel_init = timed(c, [&] {
-    launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
-    if (variant != NUC34_DIRECT)
-      launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(), c->s);
+    if (variant == NUC34_DIRECT) {
+      launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
+      return;
+    }
+    {
+      DevBuf cnt = c->alloc.get(c->WA * 4 + 4, "peel row counts");
+      DevBuf P = c->alloc.get((c->WA + 1) * 8, "peel row ranks");
+      DevBuf scratch = c->alloc.get(scan_scratch_elems(c->WA) * 8, "scan scratch");
+      launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(),
+                       cnt.as<uint32_t>(), c->s);
+      exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), P.as<uint64_t>(), c->WA, scratch.as<uint64_t>(), c->s);
+      launch_peel_ranks(v.PA, P.as<uint64_t>(), c->WA, c->s);
+    }
+    // the supports in peel order go through the second frontier buffer, which
+    // the rounds only use after the init
+    NUC34_HIP(hipMemsetAsync(pbad.as<void>(), 0, 8, c->s));
+    launch_peel_order(v, c->sup.as<int32_t>(), pmap.as<uint32_t>(), ptriA.as<uint32_t>(), ptriBC.as<uint64_t>(),
+                      reinterpret_cast<int32_t*>(pb.buf1), pcslot.as<uint2>(), pbad.as<uint32_t>(), c->s);
+    v.triA = ptriA.as<uint32_t>();
+    v.triBC = ptriBC.as<uint64_t>();
+    v.cSlot = pcslot.as<uint2>();
+    launch_peel_init(pb, reinterpret_cast<const int32_t*>(pb.buf1), c->s);
   });
   if (by_graph) {
     c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
@@ -730,6 +759,10 @@ void run_peel(nuc34_ctx* c, int variant) {
   });
   }
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
+  if (variant != NUC34_DIRECT) {
+    c->st.ms_peel += timed(c, [&] { launch_gather_k(pkv.as<int32_t>(), pmap.as<uint32_t>(), T, c->kv.as<int32_t>(), c->s); });
+    if (read_scalar<uint32_t>(c, pbad.as<void>())) throw std::logic_error("peel: a triangle is missing from its b-edge's row");
+  }
   c->st.rounds = h.round;
   c->st.k_max = h.level;
   c->st.unpeeled = T - h.peeled;

@@ -124,11 +124,12 @@ __device__ __forceinline__ uint32_t bits_below(uint32_t p, uint32_t base) {
 
 // LIVE: the K4 bits come from the live rows (the on-demand peel), without the
 // K4s that a partner on t's own rows owns: a partner dying in this round with
-// a lower triangle index. Triangles sharing their lowest vertex u are ordered
-// by the positions in N+(u) of their highest, then middle vertex, so for t at
-// (jw, jv) and x at bit position jx of an up word:
-//   T{u,v,x} (row b) precedes t iff  v < x ? jx < jw : jv < jw;
-//   T{u,w,x} (row a) precedes t iff  w < x ? jx < jw : jx < jv;
+// a lower triangle index. The on-demand peel numbers triangles in peel order:
+// by b-edge, then a-edge, i.e. those sharing their lowest vertex u by the
+// positions in N+(u) of their middle, then highest vertex, so for t at
+// (jv, jw) and x at bit position jx of an up word:
+//   T{u,v,x} (row b) precedes t iff  v < x ? jx < jw : jx < jv;
+//   T{u,w,x} (row a) precedes t iff  w < x ? jw < jv : jx < jv;
 // and for a down word (x at position i of N-(u), x below u) both row partners
 // have lowest vertex x, and precede t iff id x < id u, i.e. i < icut[u].
 template <bool LIVE>
@@ -137,19 +138,22 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
   if (wi < r.W) {
     const uint64_t ra = r.ru0 + static_cast<uint64_t>(r.jw) * r.W + wi;
     const uint64_t rb = r.ru0 + static_cast<uint64_t>(r.jv) * r.W + wi;
-    k.Aa = s.rows.Aout[ra];
-    k.Ab = s.rows.Aout[rb];
     if (LIVE) {
+      // peel rows: Aout bits with their peel-order rank, so one load per row
       const PeelWord la = s.PA[ra], lb = s.PA[rb];
+      k.Aa = la.bits;
+      k.Ab = lb.bits;
       k.Ia = RankWord{la.bits, la.rank};
       k.Ib = RankWord{lb.bits, lb.rank};
       const uint32_t bw = bits_below(r.jw, wi * 32), bv = bits_below(r.jv, wi * 32);
-      const uint32_t lower_b = (k.Ab & bw) | (~k.Ab & (r.jv < r.jw ? ~0u : 0u));
-      const uint32_t lower_a = (k.Aa & bw) | (~k.Aa & bv);
+      const uint32_t lower_b = (k.Ab & bw) | (~k.Ab & bv);
+      const uint32_t lower_a = (k.Aa & (r.jw < r.jv ? ~0u : 0u)) | (~k.Aa & bv);
       k.bits = la.live & lb.live & ~((lb.dying & lower_b) | (la.dying & lower_a));
       k.Da = la.dying;
       k.Db = lb.dying;
     } else {
+      k.Aa = s.rows.Aout[ra];
+      k.Ab = s.rows.Aout[rb];
       k.Ia = s.rows.Ain[ra];
       k.Ib = s.rows.Ain[rb];
       k.bits = (k.Aa | k.Ia.bits) & (k.Ab | k.Ib.bits);
@@ -186,20 +190,16 @@ __device__ __forceinline__ uint32_t peel_rank(const PeelWord* row, uint32_t j, u
   return w.rank + static_cast<uint32_t>(__popc(w.bits & ((1u << b) - 1u)));
 }
 
-// Rank of bit position j of the a-role row at word offset `row`: from the
-// peel rows with LIVE (the on-demand peel), from Ain elsewhere.
-template <bool LIVE>
-__device__ __forceinline__ uint32_t a_rank(const K4Src& s, uint64_t row, uint32_t j) {
-  if (LIVE) {
-    const PeelWord& w = s.PA[row + (j >> 5)];
-    return rank_in_word(RankWord{w.bits, w.rank}, j & 31u);
-  }
-  return row_rank(s.rows.Ain + row, j);
+// Rank of bit position j of the peel row at word offset `row` (Aout bits, peel
+// order): the peel-order id of the triangle whose b-edge owns the row.
+__device__ __forceinline__ uint32_t b_rank(const K4Src& s, uint64_t row, uint32_t j) {
+  const PeelWord& w = s.PA[row + (j >> 5)];
+  return rank_in_word(RankWord{w.bits, w.rank}, j & 31u);
 }
 
-// The three partner triangles of the K4 at bit j of word wi. With LIVE, *s2
-// gets q[2]'s live and dying planes (peel_rank) from the word its rank comes
-// from.
+// The three partner triangles of the K4 at bit j of word wi. With LIVE (peel
+// order ids from the peel rows), *s2 gets q[2]'s live and dying planes
+// (peel_rank) from the word its rank comes from.
 template <bool LIVE>
 __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, uint32_t wi, const K4Word& k,
                                             uint32_t j, uint32_t q[3], uint32_t* s2) {
@@ -207,24 +207,47 @@ __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, ui
     const uint32_t jx = wi * 32 + j;
     const bool vx = (k.Ab >> j) & 1u;  // v ranks below x
     const bool wx = (k.Aa >> j) & 1u;  // w ranks below x
-    const uint64_t ix = r.ru0 + static_cast<uint64_t>(jx) * r.W;  // a-role row of (u, x)
-    q[0] = vx ? a_rank<LIVE>(s, ix, r.jv) : rank_in_word(k.Ib, j);  // T{u,v,x}
-    q[1] = wx ? a_rank<LIVE>(s, ix, r.jw) : rank_in_word(k.Ia, j);  // T{u,w,x}
-    const uint32_t evx = static_cast<uint32_t>(s.triBC[q[0]]);  // c-edge of T{u,v,x}: v-x
+    const uint64_t ix = r.ru0 + static_cast<uint64_t>(jx) * r.W;  // row of (u, x)
     uint64_t row2;
     uint32_t pos2;
-    if (vx) {
-      row2 = r.rv0 + static_cast<uint64_t>(wx ? evx - r.ov : r.c - r.ov) * r.Wv;
-      pos2 = wx ? r.c - r.ov : evx - r.ov;  // v<w<x: a=(v,x), b=(v,w); v<x<w: a=(v,w), b=(v,x)
+    if constexpr (LIVE) {
+      // An id is the rank in the row of the triangle's b-edge, at its highest
+      // vertex. x above v (w): T{u,v,x} (T{u,w,x}) sits at bit j of this word;
+      // neighbouring lanes take consecutive ids. x below: row (u, x).
+      q[0] = vx ? rank_in_word(k.Ib, j) : b_rank(s, ix, r.jv);  // T{u,v,x}
+      q[1] = wx ? rank_in_word(k.Ia, j) : b_rank(s, ix, r.jw);  // T{u,w,x}
+      const uint32_t e0 = static_cast<uint32_t>(s.triBC[q[0]]);  // c-edge of T{u,v,x}: v-x (x-v if x<v)
+      if (vx) {
+        // v<w<x: b = (v,w) = t's c-edge, x's position in N+(v);
+        // v<x<w: b = (v,x), w's position in N+(v)
+        row2 = r.rv0 + static_cast<uint64_t>(wx ? r.c - r.ov : e0 - r.ov) * r.Wv;
+        pos2 = wx ? e0 - r.ov : r.c - r.ov;
+      } else {
+        // x<v<w: b = (x,v), w's position in N+(x) from the edge id of (x,w),
+        // the c-edge of T{u,x,w}
+        const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);
+        const uint32_t x = s.E[r.ou + jx];
+        row2 = s.rows.rowA[e0];
+        pos2 = exw - s.O[x];
+      }
+      q[2] = peel_rank(s.PA + row2, pos2, s2);
     } else {
-      // x<v<w: a=(x,w), b=(x,v); x = N+(u)[jx] (consecutive for neighbouring
-      // lanes), v's position in N+(x) from the edge id of (x,v)
-      const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);
-      const uint32_t x = s.E[r.ou + jx];
-      row2 = s.rows.rowA[exw];
-      pos2 = evx - s.O[x];
+      q[0] = vx ? row_rank(s.rows.Ain + ix, r.jv) : rank_in_word(k.Ib, j);  // T{u,v,x}
+      q[1] = wx ? row_rank(s.rows.Ain + ix, r.jw) : rank_in_word(k.Ia, j);  // T{u,w,x}
+      const uint32_t evx = static_cast<uint32_t>(s.triBC[q[0]]);  // c-edge of T{u,v,x}: v-x
+      if (vx) {
+        row2 = r.rv0 + static_cast<uint64_t>(wx ? evx - r.ov : r.c - r.ov) * r.Wv;
+        pos2 = wx ? r.c - r.ov : evx - r.ov;  // v<w<x: a=(v,x), b=(v,w); v<x<w: a=(v,w), b=(v,x)
+      } else {
+        // x<v<w: a=(x,w), b=(x,v); x = N+(u)[jx] (consecutive for neighbouring
+        // lanes), v's position in N+(x) from the edge id of (x,v)
+        const uint32_t exw = static_cast<uint32_t>(s.triBC[q[1]]);
+        const uint32_t x = s.E[r.ou + jx];
+        row2 = s.rows.rowA[exw];
+        pos2 = evx - s.O[x];
+      }
+      q[2] = row_rank(s.rows.Ain + row2, pos2);
     }
-    q[2] = LIVE ? peel_rank(s.PA + row2, pos2, s2) : row_rank(s.rows.Ain + row2, pos2);
   } else {
     const uint2 xuv = s.cSlot[rank_in_word(k.Ib, j)];  // T{x,u,v}: c-edge b
     q[0] = xuv.x;
@@ -1241,13 +1264,17 @@ __global__ __launch_bounds__(kBlock) void k_residual_sum(const int32_t* __restri
   }
 }
 
+// Peel rows: PA holds the Aout bits (their ranks follow from a scan of cnt),
+// the live plane over Aout | Ain and an empty dying plane; PC the Cm words.
 __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* __restrict__ IO,
-                            const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut) {
+                            const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut,
+                            uint32_t* __restrict__ cnt) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (uint64_t i = tid; i < WA; i += stride) {
-    const RankWord a = s.rows.Ain[i];
-    s.PA[i] = PeelWord{a.bits, a.rank, s.rows.Aout[i] | a.bits, 0u};
+    const uint32_t o = s.rows.Aout[i];
+    s.PA[i] = PeelWord{o, 0u, o | s.rows.Ain[i].bits, 0u};
+    cnt[i] = static_cast<uint32_t>(__popc(o));
   }
   for (uint64_t i = tid; i < WC; i += stride) {
     const RankWord c = s.rows.Cm[i];
@@ -1255,6 +1282,54 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* _
   }
   for (uint64_t u = tid; u < n; u += stride)  // in-lists are sorted by source id
     icut[u] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(IE, IO[u], IO[u + 1], static_cast<uint32_t>(u)) - IO[u]);
+}
+
+__global__ void k_peel_ranks(PeelWord* __restrict__ w, const uint64_t* __restrict__ P, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    w[i].rank = static_cast<uint32_t>(P[i]);
+}
+
+// Peel order: triangle i of the canonical order gets id map[i], the rank of
+// its bit in the peel row of its b-edge (at its highest vertex). Its edges and
+// support move to that id. A bit found clear sets *bad (no write).
+__global__ __launch_bounds__(256) void k_peel_order(K4Src s, const int32_t* __restrict__ sup,
+                                                    uint32_t* __restrict__ map, uint32_t* __restrict__ triA2,
+                                                    uint64_t* __restrict__ triBC2, int32_t* __restrict__ sup2,
+                                                    uint32_t* __restrict__ bad) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < s.T;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t a = s.triA[i];
+    const uint64_t bc = s.triBC[i];
+    const uint32_t b = static_cast<uint32_t>(bc >> 32);
+    const uint32_t jw = a - s.O[s.src[b]];
+    const PeelWord w = s.PA[s.rows.rowA[b] + (jw >> 5)];
+    const uint32_t id = rank_in_word(RankWord{w.bits, w.rank}, jw & 31u);
+    if (!((w.bits >> (jw & 31u)) & 1u) || id >= s.T) {
+      atomic_or_u32(bad, 1u);
+      continue;
+    }
+    map[i] = id;
+    triA2[id] = a;
+    triBC2[id] = bc;
+    sup2[id] = sup[i];
+  }
+}
+
+__global__ void k_cslot_order(const uint2* __restrict__ cs, const uint32_t* __restrict__ map, uint64_t T,
+                              uint2* __restrict__ out) {
+  for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < T;
+       r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint2 e = cs[r];
+    out[r] = make_uint2(map[e.x], e.y);
+  }
+}
+
+__global__ void k_gather_k(const int32_t* __restrict__ kp, const uint32_t* __restrict__ map, uint64_t T,
+                           int32_t* __restrict__ kv) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    kv[i] = kp[map[i]];
 }
 
 }  // namespace
@@ -1303,10 +1378,32 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
 }
 
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
-                      uint32_t* icut, hipStream_t s) {
+                      uint32_t* icut, uint32_t* cnt, hipStream_t s) {
   const uint64_t most = WA > WC ? (WA > n ? WA : n) : (WC > n ? WC : n);
   if (!most) return;
-  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC, IO, IE, n, icut);
+  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC, IO, IE, n, icut,
+                     cnt);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_peel_ranks(PeelWord* PA, const uint64_t* P, uint64_t WA, hipStream_t s) {
+  if (!WA) return;
+  hipLaunchKernelGGL(k_peel_ranks, dim3(grid_for(WA, 256, 1u << 16)), dim3(256), 0, s, PA, P, WA);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_peel_order(const K4Src& src, const int32_t* sup, uint32_t* map, uint32_t* triA2, uint64_t* triBC2,
+                       int32_t* sup2, uint2* cslot2, uint32_t* bad, hipStream_t s) {
+  if (!src.T) return;
+  const unsigned grid = grid_for(src.T, 256, 1u << 16);
+  hipLaunchKernelGGL(k_peel_order, dim3(grid), dim3(256), 0, s, src, sup, map, triA2, triBC2, sup2, bad);
+  hipLaunchKernelGGL(k_cslot_order, dim3(grid), dim3(256), 0, s, src.cSlot, map, src.T, cslot2);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_gather_k(const int32_t* kp, const uint32_t* map, uint64_t T, int32_t* kv, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_gather_k, dim3(grid_for(T, 256, 1u << 16)), dim3(256), 0, s, kp, map, T, kv);
   NUC34_LAUNCH_CHECK();
 }
 
