@@ -618,6 +618,21 @@ void run_peel(nuc34_ctx* c, int variant) {
   const int per_launch = c->tune.peel_rounds_per_launch ? static_cast<int>(c->tune.peel_rounds_per_launch & ~1u)
                                                         : kRoundsPerLaunch;
   const PeelShape shape = peel_shape(c);
+  // push regions of the debit workgroups: room for twice their even share of
+  // the triangles, 32 to 1024 slots each (a fuller workgroup overflows to the
+  // shared list, which stays correct)
+  DevBuf wslots, wcnt;
+  if (!by_graph) {
+    const uint64_t nwg = shape.debit_grid;
+    const uint64_t share = (2 * T + nwg - 1) / nwg;
+    const uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(32, (share + 31) & ~31ull)));
+    wslots = c->alloc.get(nwg * cap * 4, "peel push regions");
+    wcnt = c->alloc.get(nwg * 4, "peel push counts");
+    pb.wslots = wslots.as<uint32_t>();
+    pb.wcnt = wcnt.as<uint32_t>();
+    pb.wcap = cap;
+    pb.nwg = static_cast<uint32_t>(nwg);
+  }
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {
       if (variant == NUC34_DIRECT)

@@ -675,6 +675,45 @@ __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[
   for (int k = 0; k < 3; ++k) sq[k] = tri_state(p.st, q[k]);
 }
 
+// Where a debit puts the survivors it pushes. A workgroup reserves slots of
+// its own region with a counter in LDS: no global counter is shared by the
+// grid, since atomics on one address serialize at memory (about 90 M/s on
+// MI355X, against some 84 M wave appends in one R-MAT s20 + 64 K30 peel). A
+// reservation that does not fit goes to the overflow list (global counter).
+struct PushSink {
+  uint32_t* region;   // the workgroup's slots
+  uint32_t cap;
+  uint32_t* cnt;      // LDS: slots reserved so far
+  uint32_t* fit;      // LDS: end of the written slots (cap until a reservation misses)
+  uint32_t* ovf;      // overflow list and its count
+  uint32_t* ovf_cnt;
+};
+
+// Appends q[k] for every bit k of pushes, one reservation per wave.
+__device__ __forceinline__ void push3(const PushSink& k, const uint32_t q[3], uint32_t pushes) {
+  const uint32_t n = static_cast<uint32_t>(__popc(pushes));
+  const uint64_t b0 = __ballot(n & 1u);
+  const uint64_t b1 = __ballot((n >> 1) & 1u);
+  const uint32_t total = static_cast<uint32_t>(__popcll(b0) + 2 * __popcll(b1));
+  if (!total) return;
+  const unsigned leader = static_cast<unsigned>(__builtin_ctzll(b0 | b1));
+  uint32_t base = 0, over = 0;
+  if (lane_id() == leader) {
+    base = atomicAdd(k.cnt, total);
+    if (base + total > k.cap) {
+      atomicMin(k.fit, base);
+      base = atomic_add_u32(k.ovf_cnt, total);
+      over = 1;
+    }
+  }
+  base = __shfl(base, static_cast<int>(leader));
+  over = __shfl(over, static_cast<int>(leader));
+  uint32_t* dst = (over ? k.ovf : k.region) + base + lanes_below(b0) + 2 * lanes_below(b1);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    if ((pushes >> j) & 1u) *dst++ = q[j];
+}
+
 // Owner test and debit for one K4 of dying triangle t with partners q[0..2]
 // in states sq[0..2]. A partner that is alive, or was pushed this round (dies
 // next round), counts as surviving for ownership; an earlier death means the
@@ -685,7 +724,7 @@ __device__ __forceinline__ void read_states(const PeelBufs& p, const uint32_t q[
 // the extra debit of a pushed triangle cannot push it again (its support is
 // already at most level).
 __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const uint32_t sq[3], const PeelBufs& p,
-                                         int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+                                         int32_t level, const PushSink& sink) {
   bool own = true;
 #pragma unroll
   for (int k = 0; k < 3; ++k) own = own && sq[k] != kGone && !(sq[k] == kCur && q[k] < t);
@@ -706,12 +745,7 @@ __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const 
       p.kv[q[k]] = level;
     }
   }
-  // one append per wave for every push of every lane: the frontier counter is
-  // a single hot address
-  uint32_t slot = wave_append3(ncount, static_cast<uint32_t>(__popc(pushes)));
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if ((pushes >> k) & 1u) nxt[slot++] = q[k];
+  push3(sink, q, pushes);
 }
 
 // On-demand variant: the K4 came from t's live rows, so q[0] and q[1] are not
@@ -724,8 +758,7 @@ __device__ __forceinline__ void debit_k4(uint32_t t, const uint32_t q[3], const 
 // be pushed again). The planes change only between rounds, so no read here
 // races with a write.
 __device__ __forceinline__ void destroy_k4_live(uint32_t t, const uint32_t q[3], uint32_t die, const PeelBufs& p,
-                                                int32_t level, uint32_t* __restrict__ nxt,
-                                                uint32_t* __restrict__ ncount) {
+                                                int32_t level, const PushSink& sink) {
   const bool live2 = (die >> 2) & 1u, dying2 = (die >> 3) & 1u;
   if (!live2 || (dying2 && q[2] < t)) return;
   bool live[3];
@@ -745,10 +778,7 @@ __device__ __forceinline__ void destroy_k4_live(uint32_t t, const uint32_t q[3],
       p.kv[q[k]] = level;
     }
   }
-  uint32_t slot = wave_append3(ncount, static_cast<uint32_t>(__popc(pushes)));
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if ((pushes >> k) & 1u) nxt[slot++] = q[k];
+  push3(sink, q, pushes);
 }
 
 // Dying triangles per wave for a frontier of n: spread over every wave of the
@@ -804,13 +834,54 @@ __device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
 // State words per thread and pass of a level scan.
 constexpr int kScanWords = 4;
 
+// A triangle pushed this round joins the next frontier: the dying planes (on
+// demand) or state 10 -> 01.
+template <bool UNLINK>
+__device__ __forceinline__ void promote(const PeelBufs& p, const K4Src& s, uint32_t t) {
+  if (UNLINK) mark_dying(s, t);
+  else atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));
+}
+
 template <bool UNLINK>
 __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
   __shared__ uint32_t wd[kScanWords][kWPB], base;
+  __shared__ uint32_t s_tot[kWPB], s_bef[kWPB], s_pre[kWave], s_run;
   if (in->done) return;
   const uint32_t F = in->ncur + in->nspec;
   const bool collect_round = F == 0;
-  const bool need = !collect_round && in->nnext == 0;
+  const unsigned wv = threadIdx.x / kWave;
+  const unsigned lane = lane_id();
+  // The round's pushes: the overflow list nxt[0, nnext), then the debit
+  // workgroups' regions in workgroup order. Every workgroup sums all region
+  // counts (the total decides the level scan below) and those before its own
+  // share of regions, [r0, r1).
+  const uint32_t nwg = p.nwg;
+  const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * nwg / gridDim.x);
+  const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * nwg / gridDim.x);
+  uint32_t tot = 0, bef = 0;
+  for (uint32_t r = threadIdx.x; r < nwg; r += kBlock) {
+    const uint32_t c = p.wcnt[r];
+    tot += c;
+    bef += r < r0 ? c : 0u;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    tot += static_cast<uint32_t>(__shfl_xor(tot, off));
+    bef += static_cast<uint32_t>(__shfl_xor(bef, off));
+  }
+  if (lane == 0) {
+    s_tot[wv] = tot;
+    s_bef[wv] = bef;
+  }
+  __syncthreads();
+  tot = bef = 0;
+  for (int i = 0; i < kWPB; ++i) {
+    tot += s_tot[i];
+    bef += s_bef[i];
+  }
+  const uint32_t novf = in->nnext;
+  const uint32_t nn = novf + tot;
+  const bool need = !collect_round && nn == 0;
   const int32_t L = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
   uint32_t* nxt = in->sel ? p.buf0 : p.buf1;
@@ -826,16 +897,38 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
     if (UNLINK) unlink_tri(s, t);
     else atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
   }
-  const uint32_t nn = in->nnext;
-  for (uint64_t i = gtid; i < nn; i += gsize) {
-    const uint32_t t = nxt[i];
-    if (UNLINK) mark_dying(s, t);
-    else atomic_xor_u32(&p.st[t >> 4], 3u << ((t & 15u) * 2u));  // 10 -> 01
+  for (uint64_t i = gtid; i < novf; i += gsize) promote<UNLINK>(p, s, nxt[i]);
+  // this workgroup's regions, 64 at a time: wave 0 places them after the
+  // overflow list and the regions before, every wave copies some
+  if (tot) {
+    if (threadIdx.x == 0) s_run = novf + bef;
+    for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
+      __syncthreads();
+      if (wv == 0) {
+        const uint32_t c = c0 + lane < r1 ? p.wcnt[c0 + lane] : 0u;
+        const uint32_t inc = wave_inclusive_sum(c);
+        const uint32_t run = s_run;
+        s_pre[lane] = run + inc - c;
+        wave_sync();
+        if (lane == kWave - 1) s_run = run + inc;
+      }
+      __syncthreads();
+      const uint32_t nr = min(static_cast<uint32_t>(kWave), r1 - c0);
+      for (uint32_t k = wv; k < nr; k += kWPB) {
+        const uint32_t c = p.wcnt[c0 + k];
+        const uint32_t* src = p.wslots + static_cast<uint64_t>(c0 + k) * p.wcap;
+        const uint32_t o = s_pre[k];
+        for (uint32_t j = lane; j < c; j += kWave) {
+          const uint32_t t = src[j];
+          nxt[o + j] = t;
+          promote<UNLINK>(p, s, t);
+        }
+      }
+    }
   }
   if (need || collect_round) {
     const int32_t take = collect_round ? L : L + 1;  // also the k of the collected triangles
     const uint64_t W = (p.T + 15) / 16;
-    const unsigned wv = threadIdx.x / kWave;
     uint32_t mn = kNoMin;
     // kScanWords state words per thread and pass (words b0 + k * kBlock +
     // thread), their loads issued together; one block prefix and one
@@ -924,6 +1017,26 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
   }
 }
 
+// The debit workgroup's push sink: its region of the round's push slots,
+// counters in LDS (s_cnt, s_fit), overflow to ovf / *ovf_cnt. Every thread of
+// the workgroup calls it.
+__device__ __forceinline__ PushSink open_sink(const PeelBufs& p, uint32_t* ovf, uint32_t* ovf_cnt, uint32_t* s_cnt,
+                                              uint32_t* s_fit) {
+  if (threadIdx.x == 0) {
+    *s_cnt = 0;
+    *s_fit = p.wcap;
+  }
+  __syncthreads();
+  return PushSink{p.wslots + static_cast<uint64_t>(blockIdx.x) * p.wcap, p.wcap, s_cnt, s_fit, ovf, ovf_cnt};
+}
+
+// Publishes the workgroup's push count (every workgroup, every round, so the
+// finish never reads a stale one). Every thread of the workgroup calls it.
+__device__ __forceinline__ void close_sink(const PeelBufs& p, const PushSink& k) {
+  __syncthreads();
+  if (threadIdx.x == 0) p.wcnt[blockIdx.x] = min(*k.cnt, *k.fit);
+}
+
 // Prepares the slot this round's finish writes: its collection count and its
 // min reduction start empty, and a finished peel marks it done too, so that
 // rounds still queued in the captured launch do nothing whichever slot they
@@ -969,7 +1082,7 @@ __device__ __forceinline__ uint32_t find_slot_u64(const uint64_t* pre, uint64_t 
 __device__ __forceinline__ void debit_lists(const PeelBufs& p, const uint32_t* __restrict__ cur, uint32_t n,
                                             uint64_t wave, uint64_t nwaves, uint32_t G, ListTile& L,
                                             const uint64_t* __restrict__ koff, const uint32_t* __restrict__ lists,
-                                            int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+                                            int32_t level, const PushSink& sink) {
   const unsigned lane = lane_id();
   for (uint64_t tile = wave; tile * G < n; tile += nwaves) {
     const uint64_t i = tile * G + lane;
@@ -1008,8 +1121,8 @@ __device__ __forceinline__ void debit_lists(const PeelBufs& p, const uint32_t* _
       }
       read_states(p, qa, sa);
       if (two) read_states(p, qb, sb);
-      debit_k4(ta, qa, sa, p, level, nxt, ncount);
-      if (two) debit_k4(tb, qb, sb, p, level, nxt, ncount);
+      debit_k4(ta, qa, sa, p, level, sink);
+      if (two) debit_k4(tb, qb, sb, p, level, sink);
     }
     wave_sync();
   }
@@ -1020,14 +1133,14 @@ __device__ __forceinline__ void debit_lists(const PeelBufs& p, const uint32_t* _
 // from the live bit rows, flattened over the lanes.
 __device__ __forceinline__ void debit_bits(const PeelBufs& p, const K4Src& s, const uint32_t* __restrict__ cur,
                                            uint32_t n, uint64_t wave, uint64_t nwaves, uint32_t G, TileLds& L,
-                                           int32_t level, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ncount) {
+                                           int32_t level, const PushSink& sink) {
   const unsigned lane = lane_id();
   for (uint64_t tile = wave; tile * G < n; tile += nwaves) {
     const uint64_t i = tile * G + lane;
     const bool valid = lane < G && i < n;
     k4_tile<false, true>(s, valid ? cur[i] : 0u, valid, 0, L,
                          [&](uint32_t tt, uint64_t, uint32_t, const uint32_t q[3], uint32_t die) {
-                           destroy_k4_live(tt, q, die, p, level, nxt, ncount);
+                           destroy_k4_live(tt, q, die, p, level, sink);
                          });
   }
 }
@@ -1048,8 +1161,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   const unsigned w = threadIdx.x / kWave;
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
-  debit_lists(p, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], koff, lists, level, nxt,
-              &in->nnext);
+  __shared__ uint32_t s_cnt, s_fit;
+  const PushSink sink = open_sink(p, nxt, &in->nnext, &s_cnt, &s_fit);
+  debit_lists(p, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], koff, lists, level, sink);
+  close_sink(p, sink);
   prepare_out_slot(out, done);
 }
 
@@ -1066,7 +1181,10 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   const unsigned w = threadIdx.x / kWave;
   const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kWPB;
   const uint32_t G = min(tile_width(n, static_cast<uint32_t>(nw)), tile_cap);
-  debit_bits(p, s, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], level, nxt, &in->nnext);
+  __shared__ uint32_t s_cnt, s_fit;
+  const PushSink sink = open_sink(p, nxt, &in->nnext, &s_cnt, &s_fit);
+  debit_bits(p, s, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], level, sink);
+  close_sink(p, sink);
   prepare_out_slot(out, done);
 }
 
@@ -1108,7 +1226,7 @@ __global__ __launch_bounds__(kGraphBlock) void k_peel_graphs(PeelBufs p, K4Src s
                                                              uint32_t tile_cap) {
   __shared__ ListTile lt[LIVE ? 1 : kGraphWaves];
   __shared__ TileLds tl[LIVE ? kGraphWaves : 1];
-  __shared__ uint32_t s_g, s_n, s_nn, s_min, s_max;
+  __shared__ uint32_t s_g, s_n, s_nn, s_fit, s_min, s_max;
   const unsigned w = threadIdx.x / kWave;
   const unsigned lane = lane_id();
   for (;;) {
@@ -1214,11 +1332,16 @@ __global__ __launch_bounds__(kGraphBlock) void k_peel_graphs(PeelBufs p, K4Src s
         if (threadIdx.x == 0) atomic_or_u32(q.error, 2u);
         break;
       }
-      if (threadIdx.x == 0) s_nn = 0;
+      if (threadIdx.x == 0) {
+        s_nn = 0;
+        s_fit = 0xFFFFFFFFu;
+      }
       __syncthreads();
       const uint32_t G = min(tile_width(n, kGraphWaves), tile_cap);
-      if constexpr (LIVE) debit_bits(p, s, cur, n, w, kGraphWaves, G, tl[w], level, nxt, &s_nn);
-      else debit_lists(p, cur, n, w, kGraphWaves, G, lt[w], koff, lists, level, nxt, &s_nn);
+      // the graph's own next buffer holds every push (at most its triangles)
+      const PushSink sink{nxt, 0xFFFFFFFFu, &s_nn, &s_fit, nxt, &s_nn};
+      if constexpr (LIVE) debit_bits(p, s, cur, n, w, kGraphWaves, G, tl[w], level, sink);
+      else debit_lists(p, cur, n, w, kGraphWaves, G, lt[w], koff, lists, level, sink);
       __syncthreads();
       const uint32_t nn = s_nn;
       // as in k_peel_finish: the on-demand peel keeps its states in the planes

@@ -115,6 +115,15 @@ struct PeelBufs {
   uint32_t* buf1;
   PeelCtl* ctl;     // two slots
   uint64_t T;
+  // A round's pushes (survivors falling to the level): workgroup w of the
+  // debit grid (nwg workgroups) writes its own to wslots[w * wcap, ...) and
+  // their count to wcnt[w]; what does not fit goes to the front of the next
+  // frontier buffer (count in the control's nnext). The finish gathers both
+  // into the next frontier. The per-graph peel does not use them.
+  uint32_t* wslots;
+  uint32_t* wcnt;
+  uint32_t wcap;
+  uint32_t nwg;
 };
 
 // One peel round at position `slot` of its group: the debit of every K4 owned
