@@ -621,17 +621,23 @@ void run_peel(nuc34_ctx* c, int variant) {
   // push regions of the debit workgroups: room for twice their even share of
   // the triangles, 32 to 1024 slots each (a fuller workgroup overflows to the
   // shared list, which stays correct)
-  DevBuf wslots, wcnt;
+  DevBuf wslots, wcnt, woff, gcnt;
   if (!by_graph) {
     const uint64_t nwg = shape.debit_grid;
     const uint64_t share = (2 * T + nwg - 1) / nwg;
     const uint32_t cap = static_cast<uint32_t>(std::min<uint64_t>(1024, std::max<uint64_t>(32, (share + 31) & ~31ull)));
     wslots = c->alloc.get(nwg * cap * 4, "peel push regions");
     wcnt = c->alloc.get(nwg * 4, "peel push counts");
+    woff = c->alloc.get(nwg * 4, "peel push offsets");
+    gcnt = c->alloc.get(2 * kPushGroups * kGroupStride * 4, "peel push group counters");
+    NUC34_HIP(hipMemsetAsync(gcnt.as<void>(), 0, gcnt.bytes(), c->s));
     pb.wslots = wslots.as<uint32_t>();
     pb.wcnt = wcnt.as<uint32_t>();
+    pb.woff = woff.as<uint32_t>();
+    pb.gcnt = gcnt.as<uint32_t>();
     pb.wcap = cap;
     pb.nwg = static_cast<uint32_t>(nwg);
+    pb.gsize = static_cast<uint32_t>((nwg + kPushGroups - 1) / kPushGroups);
   }
   auto enqueue_rounds = [&](int n) {
     for (int r = 0; r < n; ++r) {

@@ -834,6 +834,24 @@ __device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
 // State words per thread and pass of a level scan.
 constexpr int kScanWords = 4;
 
+// Exclusive scan of v over the workgroup (kBlock threads) into *excl; returns
+// the total. Every thread calls it; sw holds kWPB words of LDS.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sw, uint32_t* excl) {
+  const uint32_t inc = wave_inclusive_sum(v);
+  const unsigned wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) sw[wv] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (unsigned i = 0; i < static_cast<unsigned>(kWPB); ++i) {
+    before += i < wv ? sw[i] : 0u;
+    total += sw[i];
+  }
+  *excl = before + inc - v;
+  __syncthreads();
+  return total;
+}
+
 // A triangle pushed this round joins the next frontier: the dying planes (on
 // demand) or state 10 -> 01.
 template <bool UNLINK>
@@ -845,42 +863,27 @@ __device__ __forceinline__ void promote(const PeelBufs& p, const K4Src& s, uint3
 template <bool UNLINK>
 __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s) {
   __shared__ uint32_t wd[kScanWords][kWPB], base;
-  __shared__ uint32_t s_tot[kWPB], s_bef[kWPB], s_pre[kWave], s_run;
+  __shared__ uint32_t s_sw[kWPB], s_gpre[kPushGroups], s_lpre[kBlock], s_base[kBlock];
+  static_assert(kPushGroups <= static_cast<uint32_t>(kBlock), "one group total per thread");
   if (in->done) return;
   const uint32_t F = in->ncur + in->nspec;
   const bool collect_round = F == 0;
   const unsigned wv = threadIdx.x / kWave;
-  const unsigned lane = lane_id();
   // The round's pushes: the overflow list nxt[0, nnext), then the debit
-  // workgroups' regions in workgroup order. Every workgroup sums all region
-  // counts (the total decides the level scan below) and those before its own
-  // share of regions, [r0, r1).
-  const uint32_t nwg = p.nwg;
-  const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * nwg / gridDim.x);
-  const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * nwg / gridDim.x);
-  uint32_t tot = 0, bef = 0;
-  for (uint32_t r = threadIdx.x; r < nwg; r += kBlock) {
-    const uint32_t c = p.wcnt[r];
-    tot += c;
-    bef += r < r0 ? c : 0u;
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    tot += static_cast<uint32_t>(__shfl_xor(tot, off));
-    bef += static_cast<uint32_t>(__shfl_xor(bef, off));
-  }
-  if (lane == 0) {
-    s_tot[wv] = tot;
-    s_bef[wv] = bef;
-  }
+  // workgroups' regions, group after group, each at its offset in its group.
+  const uint32_t par = static_cast<uint32_t>(in - p.ctl);
+  uint32_t* const gc = p.gcnt + par * kPushGroups * kGroupStride;
+  const uint32_t ngroups = (p.nwg + p.gsize - 1) / p.gsize;
+  uint32_t gpre;
+  const uint32_t gtot =
+      block_exclusive_scan(threadIdx.x < ngroups ? gc[threadIdx.x * kGroupStride] : 0u, s_sw, &gpre);
+  if (threadIdx.x < kPushGroups) s_gpre[threadIdx.x] = gpre;
+  // the next round's counters (the other parity) were read by the round
+  // before this one; they start the next round at zero
+  if (blockIdx.x == 0 && threadIdx.x < ngroups) p.gcnt[((par ^ 1u) * kPushGroups + threadIdx.x) * kGroupStride] = 0;
   __syncthreads();
-  tot = bef = 0;
-  for (int i = 0; i < kWPB; ++i) {
-    tot += s_tot[i];
-    bef += s_bef[i];
-  }
   const uint32_t novf = in->nnext;
-  const uint32_t nn = novf + tot;
+  const uint32_t nn = novf + gtot;
   const bool need = !collect_round && nn == 0;
   const int32_t L = round_level(in);
   const uint32_t* cur = in->sel ? p.buf1 : p.buf0;
@@ -898,32 +901,34 @@ __global__ __launch_bounds__(kBlock) void k_peel_finish(PeelBufs p, PeelCtl* in,
     else atomic_or_u32(&p.st[t >> 4], 2u << ((t & 15u) * 2u));  // 01 -> 11
   }
   for (uint64_t i = gtid; i < novf; i += gsize) promote<UNLINK>(p, s, nxt[i]);
-  // this workgroup's regions, 64 at a time: wave 0 places them after the
-  // overflow list and the regions before, every wave copies some
-  if (tot) {
-    if (threadIdx.x == 0) s_run = novf + bef;
-    for (uint32_t c0 = r0; c0 < r1; c0 += kWave) {
+  if (gtot) {
+    // this workgroup's regions [r0, r1), kBlock at a time, their pushes
+    // flattened over the threads
+    const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x) * p.nwg / gridDim.x);
+    const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(blockIdx.x + 1) * p.nwg / gridDim.x);
+    for (uint32_t c0 = r0; c0 < r1; c0 += kBlock) {
+      const uint32_t r = c0 + threadIdx.x;
+      const bool in_range = r < r1;
+      const uint32_t c = in_range ? p.wcnt[r] : 0u;
+      const uint32_t b = in_range && c ? novf + s_gpre[r / p.gsize] + p.woff[r] : 0u;
+      uint32_t lp;
+      const uint32_t sum = block_exclusive_scan(c, s_sw, &lp);
+      s_lpre[threadIdx.x] = lp;
+      s_base[threadIdx.x] = b;
       __syncthreads();
-      if (wv == 0) {
-        const uint32_t c = c0 + lane < r1 ? p.wcnt[c0 + lane] : 0u;
-        const uint32_t inc = wave_inclusive_sum(c);
-        const uint32_t run = s_run;
-        s_pre[lane] = run + inc - c;
-        wave_sync();
-        if (lane == kWave - 1) s_run = run + inc;
-      }
-      __syncthreads();
-      const uint32_t nr = min(static_cast<uint32_t>(kWave), r1 - c0);
-      for (uint32_t k = wv; k < nr; k += kWPB) {
-        const uint32_t c = p.wcnt[c0 + k];
-        const uint32_t* src = p.wslots + static_cast<uint64_t>(c0 + k) * p.wcap;
-        const uint32_t o = s_pre[k];
-        for (uint32_t j = lane; j < c; j += kWave) {
-          const uint32_t t = src[j];
-          nxt[o + j] = t;
-          promote<UNLINK>(p, s, t);
+      for (uint32_t e = threadIdx.x; e < sum; e += kBlock) {
+        uint32_t lo = 0, hi = kBlock;  // the last region starting at or before e
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_lpre[mid] <= e) lo = mid;
+          else hi = mid;
         }
+        const uint32_t j = e - s_lpre[lo];
+        const uint32_t t = p.wslots[static_cast<uint64_t>(c0 + lo) * p.wcap + j];
+        nxt[s_base[lo] + j] = t;
+        promote<UNLINK>(p, s, t);
       }
+      __syncthreads();
     }
   }
   if (need || collect_round) {
@@ -1031,10 +1036,18 @@ __device__ __forceinline__ PushSink open_sink(const PeelBufs& p, uint32_t* ovf, 
 }
 
 // Publishes the workgroup's push count (every workgroup, every round, so the
-// finish never reads a stale one). Every thread of the workgroup calls it.
-__device__ __forceinline__ void close_sink(const PeelBufs& p, const PushSink& k) {
+// finish never reads a stale one) and, with pushes, takes its region's offset
+// in its group from the counter of the round's parity. Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void close_sink(const PeelBufs& p, const PushSink& k, const PeelCtl* in) {
   __syncthreads();
-  if (threadIdx.x == 0) p.wcnt[blockIdx.x] = min(*k.cnt, *k.fit);
+  if (threadIdx.x == 0) {
+    const uint32_t c = min(*k.cnt, *k.fit);
+    const uint32_t par = static_cast<uint32_t>(in - p.ctl);
+    p.wcnt[blockIdx.x] = c;
+    p.woff[blockIdx.x] =
+        c ? atomic_add_u32(p.gcnt + (par * kPushGroups + blockIdx.x / p.gsize) * kGroupStride, c) : 0u;
+  }
 }
 
 // Prepares the slot this round's finish writes: its collection count and its
@@ -1164,7 +1177,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
   __shared__ uint32_t s_cnt, s_fit;
   const PushSink sink = open_sink(p, nxt, &in->nnext, &s_cnt, &s_fit);
   debit_lists(p, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], koff, lists, level, sink);
-  close_sink(p, sink);
+  close_sink(p, sink, in);
   prepare_out_slot(out, done);
 }
 
@@ -1184,7 +1197,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_bits(PeelBufs p, PeelCtl*
   __shared__ uint32_t s_cnt, s_fit;
   const PushSink sink = open_sink(p, nxt, &in->nnext, &s_cnt, &s_fit);
   debit_bits(p, s, cur, n, static_cast<uint64_t>(blockIdx.x) * kWPB + w, nw, G, lds[w], level, sink);
-  close_sink(p, sink);
+  close_sink(p, sink, in);
   prepare_out_slot(out, done);
 }
 

@@ -122,9 +122,18 @@ struct PeelBufs {
   // into the next frontier. The per-graph peel does not use them.
   uint32_t* wslots;
   uint32_t* wcnt;
+  uint32_t* woff;   // offset of region w among its group's (the group counter's old value)
+  uint32_t* gcnt;   // push group counters: 2 round parities x kPushGroups, kGroupStride apart
   uint32_t wcap;
   uint32_t nwg;
+  uint32_t gsize;   // regions per group: ceil(nwg / kPushGroups)
 };
+// The debit workgroups' push regions form kPushGroups groups of consecutive
+// workgroups; each workgroup with pushes takes its offset from its group's
+// counter (one atomic, on a line of its own), so the finish scans only the
+// group totals.
+constexpr uint32_t kPushGroups = 256;
+constexpr uint32_t kGroupStride = 32;
 
 // One peel round at position `slot` of its group: the debit of every K4 owned
 // by a dying triangle, then the state advance and, when the frontier ran
