@@ -86,7 +86,7 @@ WORKLOADS = {
 # Launch shapes chosen by nuc34_autotune on one MI355X (bench.py --autotune
 # prints the tuner's choice); shapes change speed, never results.
 TUNED = {
-    "rmat20k30": {"peel_debit_grid": 16384, "peel_tile_cap": 8, "peel_rounds_per_launch": 64, "tri_tile_edges": 16,
+    "rmat20k30": {"peel_debit_grid": 8192, "peel_tile_cap": 8, "peel_rounds_per_launch": 64, "tri_tile_edges": 16,
                   "peel_finish_grid": 1024, "tri_schedule": 1},
     "rmat16": {"peel_debit_grid": 4096, "peel_tile_cap": 16, "peel_rounds_per_launch": 32, "k4_list_grid_cap": 8192,
                "tri_tile_edges": 64, "peel_finish_grid": 512, "tri_schedule": 1},
