@@ -63,6 +63,7 @@ struct nuc34_ctx {
   // direct: per-triangle K4 partner lists
   bool has_k4 = false;
   DevBuf koff, lists;
+  DevBuf pmap, psup;  // the lists' peel order: canonical -> peel id, supports in peel order
   // peel result
   bool has_k = false;
   DevBuf kv;
@@ -309,7 +310,7 @@ void reset_stages(nuc34_ctx* c, int from) {
     c->batch_rounds.clear();
   }
   if (from <= 3) {
-    for (DevBuf* b : {&c->koff, &c->lists}) b->release();
+    for (DevBuf* b : {&c->koff, &c->lists, &c->pmap, &c->psup}) b->release();
     c->has_k4 = false;
   }
   if (from <= 2) {
@@ -483,21 +484,75 @@ void run_support(nuc34_ctx* c) {
   c->st.four_cliques = c->Q;
 }
 
+// Peel order: triangles numbered by b-edge, then a-edge (the running ranks of
+// the Aout bits), so that a dying triangle's K4 partners above it sit at
+// consecutive ids along its own rows and the peel's decrements share cache
+// lines. o.v enumerates K4s from the peel rows (live plane over Aout | Ain,
+// dying plane empty) and names partners by peel id; o.map takes canonical
+// ids to peel ids, o.sup holds the supports in peel order.
+struct PeelOrder {
+  DevBuf liveA, liveC, icut, map, triA, triBC, cslot, sup, bad;
+  K4Src v{};
+};
+
+void build_peel_order(nuc34_ctx* c, PeelOrder& o) {
+  const uint64_t T = c->T;
+  o.v = k4src(c);
+  o.liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
+  o.liveC = c->alloc.get(c->WC * sizeof(PeelWord) + sizeof(PeelWord), "peel c rows");
+  o.icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
+  o.v.PA = o.liveA.as<PeelWord>();
+  o.v.PC = o.liveC.as<PeelWord>();
+  o.v.icut = o.icut.as<uint32_t>();
+  {
+    DevBuf cnt = c->alloc.get(c->WA * 4 + 4, "peel row counts");
+    DevBuf P = c->alloc.get((c->WA + 1) * 8, "peel row ranks");
+    DevBuf scratch = c->alloc.get(scan_scratch_elems(c->WA) * 8, "scan scratch");
+    launch_live_rows(o.v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, o.icut.as<uint32_t>(),
+                     cnt.as<uint32_t>(), c->s);
+    exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), P.as<uint64_t>(), c->WA, scratch.as<uint64_t>(), c->s);
+    launch_peel_ranks(o.v.PA, P.as<uint64_t>(), c->WA, c->s);
+  }
+  o.map = c->alloc.get(T * 4, "peel order of the triangles");
+  o.triA = c->alloc.get(T * 4, "triangles in peel order");
+  o.triBC = c->alloc.get(T * 8, "triangles in peel order");
+  o.cslot = c->alloc.get(T * 8 + 8, "c-slots in peel order");
+  o.sup = c->alloc.get(T * 4 + 4, "supports in peel order");
+  o.bad = c->alloc.get(8, "peel order check");
+  NUC34_HIP(hipMemsetAsync(o.bad.as<void>(), 0, 8, c->s));
+  launch_peel_order(o.v, c->sup.as<int32_t>(), o.map.as<uint32_t>(), o.triA.as<uint32_t>(), o.triBC.as<uint64_t>(),
+                    o.sup.as<int32_t>(), o.cslot.as<uint2>(), o.bad.as<uint32_t>(), c->s);
+  o.v.triA = o.triA.as<uint32_t>();
+  o.v.triBC = o.triBC.as<uint64_t>();
+  o.v.cSlot = o.cslot.as<uint2>();
+}
+
+void check_peel_order(nuc34_ctx* c, const PeelOrder& o) {
+  if (read_scalar<uint32_t>(c, o.bad.as<void>())) throw std::logic_error("peel order: a triangle is missing from its b-edge's row");
+}
+
+// Direct variant: every triangle's K4 list (three partners per K4), in peel
+// order: list offsets from the supports in peel order, partners by peel id.
 void run_k4_lists(nuc34_ctx* c) {
   const uint64_t T = c->T, Q = c->Q;
   c->st.ms_k4_list = timed(c, [&] {
+    PeelOrder o;
+    build_peel_order(c, o);
     c->koff = c->alloc.get((T + 1) * 8, "K4 list offsets");
     {
       DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<int32_t, uint64_t>(c->sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
+      exclusive_scan<int32_t, uint64_t>(o.sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
                                         c->s);
     }
+    check_peel_order(c, o);
     if (read_scalar<uint64_t>(c, c->koff.as<uint64_t>() + T) != 4 * Q)
       throw std::logic_error("K4 list sizes disagree with the support sum");
     c->lists = c->alloc.get(4 * Q * 12 + 12, "K4 partner lists");
     ktime(c, K_K4_LISTS, 1, [&] {
-      launch_k4_lists(k4src(c), c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->tune.k4_list_grid_cap, c->s);
+      launch_k4_lists(o.v, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->tune.k4_list_grid_cap, true, c->s);
     });
+    c->pmap = std::move(o.map);
+    c->psup = std::move(o.sup);
   });
   c->has_k4 = true;
 }
@@ -562,26 +617,14 @@ void run_peel(nuc34_ctx* c, int variant) {
   DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
               b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
-  K4Src v = k4src(c);
-  // On demand: the peel numbers triangles in peel order (b-edge, then a-edge;
-  // the ranks of the peel rows), with triangles, c-slots, supports and k in
-  // that order, and k goes back to the canonical order at the end (pmap).
-  DevBuf liveA, liveC, icut, pmap, ptriA, ptriBC, pcslot, pkv, pbad;
-  if (variant != NUC34_DIRECT) {
-    liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
-    liveC = c->alloc.get(c->WC * sizeof(PeelWord) + sizeof(PeelWord), "peel c rows");
-    icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
-    pmap = c->alloc.get(T * 4, "peel order of the triangles");
-    ptriA = c->alloc.get(T * 4, "triangles in peel order");
-    ptriBC = c->alloc.get(T * 8, "triangles in peel order");
-    pcslot = c->alloc.get(T * 8 + 8, "c-slots in peel order");
-    pkv = c->alloc.get(T * 4 + 4, "k values in peel order");
-    pbad = c->alloc.get(8, "peel order check");
-    v.PA = liveA.as<PeelWord>();
-    v.PC = liveC.as<PeelWord>();
-    v.icut = icut.as<uint32_t>();
-    pb.kv = pkv.as<int32_t>();
-  }
+  // Both variants peel in peel order (supports, states, frontiers and k by
+  // peel id); k goes back to the canonical order at the end. On demand the
+  // order and its peel rows are built here, fresh planes for every peel; the
+  // direct variant's came with its K4 lists.
+  K4Src v{};
+  PeelOrder o;
+  DevBuf pkv = c->alloc.get(T * 4 + 4, "k values in peel order");
+  pb.kv = pkv.as<int32_t>();
   // A loaded batch whose graphs are all small: one workgroup per graph, the
   // whole peel one launch (k_peel_graphs). The graphs' triangle slices come
   // from offA at the graphs' first edges; workgroups take the largest first.
@@ -651,27 +694,12 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl h{};
   c->st.ms_peel_init = timed(c, [&] {
     if (variant == NUC34_DIRECT) {
-      launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
+      launch_peel_init(pb, c->psup.as<int32_t>(), c->s);
       return;
     }
-    {
-      DevBuf cnt = c->alloc.get(c->WA * 4 + 4, "peel row counts");
-      DevBuf P = c->alloc.get((c->WA + 1) * 8, "peel row ranks");
-      DevBuf scratch = c->alloc.get(scan_scratch_elems(c->WA) * 8, "scan scratch");
-      launch_live_rows(v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, icut.as<uint32_t>(),
-                       cnt.as<uint32_t>(), c->s);
-      exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), P.as<uint64_t>(), c->WA, scratch.as<uint64_t>(), c->s);
-      launch_peel_ranks(v.PA, P.as<uint64_t>(), c->WA, c->s);
-    }
-    // the supports in peel order go through the second frontier buffer, which
-    // the rounds only use after the init
-    NUC34_HIP(hipMemsetAsync(pbad.as<void>(), 0, 8, c->s));
-    launch_peel_order(v, c->sup.as<int32_t>(), pmap.as<uint32_t>(), ptriA.as<uint32_t>(), ptriBC.as<uint64_t>(),
-                      reinterpret_cast<int32_t*>(pb.buf1), pcslot.as<uint2>(), pbad.as<uint32_t>(), c->s);
-    v.triA = ptriA.as<uint32_t>();
-    v.triBC = ptriBC.as<uint64_t>();
-    v.cSlot = pcslot.as<uint2>();
-    launch_peel_init(pb, reinterpret_cast<const int32_t*>(pb.buf1), c->s);
+    build_peel_order(c, o);
+    v = o.v;
+    launch_peel_init(pb, o.sup.as<int32_t>(), c->s);
   });
   if (by_graph) {
     c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
@@ -780,9 +808,10 @@ void run_peel(nuc34_ctx* c, int variant) {
   });
   }
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
-  if (variant != NUC34_DIRECT) {
-    c->st.ms_peel += timed(c, [&] { launch_gather_k(pkv.as<int32_t>(), pmap.as<uint32_t>(), T, c->kv.as<int32_t>(), c->s); });
-    if (read_scalar<uint32_t>(c, pbad.as<void>())) throw std::logic_error("peel: a triangle is missing from its b-edge's row");
+  {
+    const DevBuf& map = variant == NUC34_DIRECT ? c->pmap : o.map;
+    c->st.ms_peel += timed(c, [&] { launch_gather_k(pkv.as<int32_t>(), map.as<uint32_t>(), T, c->kv.as<int32_t>(), c->s); });
+    if (variant != NUC34_DIRECT) check_peel_order(c, o);
   }
   c->st.rounds = h.round;
   c->st.k_max = h.level;

@@ -492,6 +492,8 @@ __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __res
   if (lane_id() == 0 && acc) atomic_add_u64(total, acc);
 }
 
+// LIVE: from the peel rows (fresh planes, so every K4), partners by peel id.
+template <bool LIVE>
 __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __restrict__ koff,
                                                      uint32_t* __restrict__ lists) {
   __shared__ TileLds lds[kWPB];
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_k4_lists(K4Src s, const uint64_t* __
     const bool valid = t < s.T;
     // the lists stream out past the cache (they are read back a round at a
     // time much later); the row words of the lookups keep it
-    k4_tile<true, false>(s, static_cast<uint32_t>(valid ? t : 0), valid, valid ? koff[t] : 0, lds[w],
+    k4_tile<true, LIVE>(s, static_cast<uint32_t>(valid ? t : 0), valid, valid ? koff[t] : 0, lds[w],
                          [&](uint32_t, uint64_t first, uint32_t pos, const uint32_t q[3], uint32_t) {
                            uint32_t* o = lists + 3 * (first + pos);
                            __builtin_nontemporal_store(q[0], o);
@@ -1483,11 +1485,13 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st) {
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, bool live,
+                     hipStream_t st) {
   if (!s.T) return;
   const uint64_t tiles = (s.T + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_k4_lists, dim3(grid_for_xcd(tiles, kWPB, grid_cap ? grid_cap : 1u << 15)), dim3(kBlock), 0, st, s,
-                     koff, lists);
+  const dim3 grid(grid_for_xcd(tiles, kWPB, grid_cap ? grid_cap : 1u << 15));
+  if (live) hipLaunchKernelGGL(k_k4_lists<true>, grid, dim3(kBlock), 0, st, s, koff, lists);
+  else hipLaunchKernelGGL(k_k4_lists<false>, grid, dim3(kBlock), 0, st, s, koff, lists);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -58,7 +58,11 @@ void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* 
 // fourth vertex ranks above t's lowest vertex first, in order of its position
 // in that vertex's out-list, then the others in order of their fourth vertex's
 // position in the lowest vertex's in-list.
-void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, hipStream_t st);
+// With `live`, s is the peel order's (peel rows with fresh planes, triangles in
+// peel order), t and the partners are peel ids and koff scans the supports in
+// peel order.
+void launch_k4_lists(const K4Src& s, const uint64_t* koff, uint32_t* lists, unsigned grid_cap, bool live,
+                     hipStream_t st);
 // Owned K4s: (t << 32 | position of the fourth vertex in N+(u)); the records
 // [first, first + count) of the list laid out at k4off[t] .. (k4off = exclusive
 // scan of own) go to k4owned[0, count), walking triangles [t_begin, t_end).
