@@ -88,8 +88,8 @@ WORKLOADS = {
 TUNED = {
     "rmat20k30": {"peel_debit_grid": 8192, "peel_tile_cap": 8, "peel_rounds_per_launch": 64, "tri_tile_edges": 16,
                   "peel_finish_grid": 1024, "tri_schedule": 1},
-    "rmat16": {"peel_debit_grid": 4096, "peel_tile_cap": 16, "peel_rounds_per_launch": 32, "k4_list_grid_cap": 8192,
-               "tri_tile_edges": 64, "peel_finish_grid": 512, "tri_schedule": 1},
+    "rmat16": {"peel_debit_grid": 2048, "peel_tile_cap": 16, "peel_rounds_per_launch": 32, "k4_list_grid_cap": 8192,
+               "tri_tile_edges": 64, "peel_finish_grid": 1024, "tri_schedule": 1},
     "batch": {"peel_debit_grid": 2048, "peel_tile_cap": 32, "peel_rounds_per_launch": 16, "k4_list_grid_cap": 32768,
               "tri_tile_edges": 64, "peel_finish_grid": 512, "batch_peel": 2, "tri_schedule": 0},
 }
