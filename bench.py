@@ -23,8 +23,9 @@ Prints one JSON line with the driver's contract, plus:
                 traffic = its HBM bytes per round from two rocprofv3 --pmc
                 child runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
                 size in the same process;
-  roofline.atomics  the peel's support decrements (device atomics) per second
-                against the measured random-atomic rate of the device;
+  roofline.atomics  the peel's atomic line requests (a third --pmc child run)
+                per second against the measured random-atomic rate of the
+                device (one line per lane), and the decrements per request;
   cpu_baseline  the reference's own host path (oracle/_ref/ref34, compiled from
                 the reference headers) on R-MAT scale 16 (the same generator at
                 a size the host path holds: about 64 bytes per four-clique), timed
@@ -164,14 +165,20 @@ def calibration_copy(mib: int) -> None:
             raise RuntimeError(f"HIP error {rc} in the calibration copy")
 
 
+# Returning atomic requests from the CUs to L2, one per 64-byte line per wave
+# instruction: what a device atomic costs, however many lanes share the line.
+ATOMIC_REQ = "TCP_TCC_ATOMIC_WITH_RET_REQ_sum"
+
+
 def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str, tuning: dict | None,
                 calib_mib: int = 1024) -> dict | None:
     """HBM bytes per launch of `group` from two rocprofv3 --pmc runs of a
     one-step bench on the same edge file (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch), each scaled by what the same counter reads for a device-to-device
-    copy of `calib_mib` MiB made first in the same process. The child runs with
-    the launch shapes of the timed run (`tuning`). None when rocprofv3 is
-    unavailable or fails."""
+    copy of `calib_mib` MiB made first in the same process, and the group's
+    returning atomic requests per step from a third run (ATOMIC_REQ). The child
+    runs with the launch shapes of the timed run (`tuning`). None when rocprofv3
+    is unavailable or fails."""
     prof = "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
@@ -186,7 +193,7 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
     # hipGraphLaunch, in the profiler's own code called from the HSA runtime,
     # so the profiled children keep away from graph launches.
     env = dict(os.environ, NUC34_NO_GRAPH="1")
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", ATOMIC_REQ):
         d = os.path.join(tmp, counter)
         print(f"[bench] rocprofv3 --pmc {counter} child run", file=sys.stderr, flush=True)
         try:
@@ -206,6 +213,9 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
         except Exception as e:  # noqa: BLE001 - report and fall back to null
             print(f"traffic: {counter} database unreadable (exit {rc}): {e}", file=sys.stderr)
             return None
+        if counter == ATOMIC_REQ:  # a count, not bytes: no calibration
+            res[counter] = sum(v for n, v in rows if any(k in n for k in GROUP_KERNELS[group]))
+            continue
         copies = [v for n, v in rows if "copyBuffer" in n]
         if not copies:
             print(f"traffic: {counter} database has no calibration copy (exit {rc}, {len(rows)} rows)", file=sys.stderr)
@@ -219,7 +229,8 @@ def run_traffic(workload: str, edges: str, group: str, launches: float, tmp: str
             "fetch_scale": res["FETCH_SIZE"][1], "write_scale": res["WRITE_SIZE"][1],
             "group_bytes_per_step": {g: res["FETCH_SIZE"][2][g] + res["WRITE_SIZE"][2][g]
                                      for g in GROUP_KERNELS if res["FETCH_SIZE"][2][g] + res["WRITE_SIZE"][2][g] > 0},
-            "calibration": f"{calib_mib} MiB device-to-device copy in the profiled process"}
+            "calibration": f"{calib_mib} MiB device-to-device copy in the profiled process",
+            "atomic_requests_per_step": res[ATOMIC_REQ]}
 
 
 def cpu_model() -> str:
@@ -418,16 +429,25 @@ def run_serving(nuc34, seconds: float, rate: float, large_share: float, seed: in
             "server": st}
 
 
-def peel_atomics(st: dict, rates: dict | None) -> dict | None:
-    """The peel's support decrements are random device-scope atomics: their rate
-    over the peel's time against the device's measured random-atomic rate."""
+def peel_atomics(st: dict, rates: dict | None, traffic: dict | None) -> dict | None:
+    """The peel's support decrements are device-scope atomics, and a device
+    atomic costs one request per 64-byte line per wave instruction: with the
+    peel rounds' request count (PMC) the rate and floor are in requests against
+    the device's measured random-atomic rate (one line per lane); without it, in
+    decrements (an upper bound on the requests)."""
     if not st.get("debits") or not st.get("ms_peel"):
         return None
-    got = st["debits"] / (st["ms_peel"] / 1e3) / 1e9
+    sec = st["ms_peel"] / 1e3
     peak = rates["atomic_gops"] if rates else None
-    return {"debits_per_step": st["debits"], "achieved": got, "unit": "G atomics/s", "peak_measured": peak,
+    req = traffic.get("atomic_requests_per_step") if traffic else None
+    n = req if req else st["debits"]
+    got = n / sec / 1e9
+    return {"debits_per_step": st["debits"], "requests_per_step": req,
+            "debits_per_request": st["debits"] / req if req else None,
+            "counted": "requests" if req else "decrements",
+            "achieved": got, "unit": "G requests/s" if req else "G atomics/s", "peak_measured": peak,
             "frac": got / peak if peak else None,
-            "floor_ms": st["debits"] / (peak * 1e9) * 1e3 if peak else None}
+            "floor_ms": n / (peak * 1e9) * 1e3 if peak else None}
 
 
 def c3_cpu_record() -> dict | None:
@@ -607,7 +627,7 @@ def main():
                      "traffic_detail": traffic,
                      "bytes_per_launch": bytes_per_launch, "ms_per_launch": per_launch_ms,
                      "launches_per_step": launches_per_step,
-                     "atomics": peel_atomics(st, rates)},
+                     "atomics": peel_atomics(st, rates, traffic)},
         "device_rates": rates,
         "other_variant": other,
         "cpu_baseline": cpu,
