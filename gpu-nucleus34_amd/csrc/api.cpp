@@ -472,9 +472,11 @@ void run_support(nuc34_ctx* c) {
     c->own = c->alloc.get(T * 4 + 4, "owned K4 counts");
     c->small = c->alloc.get(64, "counters");
     zero(c, c->small);
+    DevBuf ab = c->alloc.get(c->WA * 8 + 8, "support rows");
+    DevBuf cb = c->alloc.get(c->WC * 4 + 4, "support rows");
     ktime(c, K_SUPPORT, 1, [&] {
-      launch_support_bits(k4src(c), c->sup.as<int32_t>(), c->own.as<uint32_t>(),
-                          reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->s);
+      launch_support_bits(k4src(c), c->WA, c->WC, ab.as<uint2>(), cb.as<uint32_t>(), c->sup.as<int32_t>(),
+                          c->own.as<uint32_t>(), reinterpret_cast<uint64_t*>(c->small.as<char>() + 8), c->s);
     });
     const uint64_t total = read_scalar<uint64_t>(c, c->small.as<char>() + 8);
     if (total % 4) throw std::logic_error("support sum is not a multiple of 4");

@@ -402,7 +402,22 @@ __device__ __forceinline__ void k4_tile(const K4Src& s, uint32_t t, bool valid, 
 // 71 ms): longer chunks spread a load instruction's lanes over more lines.
 constexpr uint32_t kSupChunk = 2;
 
-__global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __restrict__ sup,
+// Compact rows for the support count: ab[i] = {Aout, Aout | Ain} bits of A-row
+// word i, cb[i] = the Cm bits of C-row word i. The count reads only bits, and
+// the ranked words would fetch their ranks beside them.
+__global__ void k_support_rows(BitRows r, uint64_t WA, uint64_t WC, uint2* __restrict__ ab,
+                               uint32_t* __restrict__ cb) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (uint64_t i = tid; i < WA; i += stride) {
+    const uint32_t o = r.Aout[i];
+    ab[i] = make_uint2(o, o | r.Ain[i].bits);
+  }
+  for (uint64_t i = tid; i < WC; i += stride) cb[i] = r.Cm[i].bits;
+}
+
+__global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, const uint2* __restrict__ abw,
+                                                         const uint32_t* __restrict__ cbw, int32_t* __restrict__ sup,
                                                          uint32_t* __restrict__ own, uint64_t* __restrict__ total) {
   __shared__ uint32_t s_pre[kWPB][kWave + 1];
   __shared__ uint32_t s_sup[kWPB][kWave];
@@ -450,19 +465,17 @@ __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __res
         // every load of the chunk is issued before any is used
         if (ci < cui) {
           const uint32_t w0 = ci * kSupChunk;
-          uint32_t oa[kSupChunk], ob[kSupChunk], ia[kSupChunk], ib[kSupChunk];
+          uint2 xa[kSupChunk], xb[kSupChunk];
 #pragma unroll
           for (uint32_t j = 0; j < kSupChunk; ++j) {
             const bool in = w0 + j < Wi;
-            oa[j] = in ? s.rows.Aout[rai + w0 + j] : 0u;
-            ob[j] = in ? s.rows.Aout[rbi + w0 + j] : 0u;
-            ia[j] = in ? s.rows.Ain[rai + w0 + j].bits : 0u;
-            ib[j] = in ? s.rows.Ain[rbi + w0 + j].bits : 0u;
+            xa[j] = in ? abw[rai + w0 + j] : make_uint2(0u, 0u);
+            xb[j] = in ? abw[rbi + w0 + j] : make_uint2(0u, 0u);
           }
 #pragma unroll
           for (uint32_t j = 0; j < kSupChunk; ++j) {
-            c += static_cast<uint32_t>(__popc((oa[j] | ia[j]) & (ob[j] | ib[j])));
-            o += static_cast<uint32_t>(__popc(oa[j] & ob[j]));
+            c += static_cast<uint32_t>(__popc(xa[j].y & xb[j].y));
+            o += static_cast<uint32_t>(__popc(xa[j].x & xb[j].x));
           }
         } else {
           const uint32_t w0 = (ci - cui) * kSupChunk;
@@ -470,8 +483,8 @@ __global__ __launch_bounds__(kBlock) void k_support_bits(K4Src s, int32_t* __res
 #pragma unroll
           for (uint32_t j = 0; j < kSupChunk; ++j) {
             const bool in = w0 + j < WCi;
-            xa[j] = in ? s.rows.Cm[cai + w0 + j].bits : 0u;
-            xb[j] = in ? s.rows.Cm[cbi + w0 + j].bits : 0u;
+            xa[j] = in ? cbw[cai + w0 + j] : 0u;
+            xb[j] = in ? cbw[cbi + w0 + j] : 0u;
           }
 #pragma unroll
           for (uint32_t j = 0; j < kSupChunk; ++j) c += static_cast<uint32_t>(__popc(xa[j] & xb[j]));
@@ -1480,10 +1493,14 @@ void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s) {
   NUC34_LAUNCH_CHECK();
 }
 
-void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st) {
+void launch_support_bits(const K4Src& s, uint64_t WA, uint64_t WC, uint2* ab, uint32_t* cb, int32_t* sup,
+                         uint32_t* own, uint64_t* total, hipStream_t st) {
   if (!s.T) return;
+  const uint64_t most = WA > WC ? WA : WC;
+  hipLaunchKernelGGL(k_support_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, st, s.rows, WA, WC, ab, cb);
   const uint64_t tiles = (s.T + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(tiles, kWPB, 1u << 14)), dim3(kBlock), 0, st, s, sup, own, total);
+  hipLaunchKernelGGL(k_support_bits, dim3(grid_for(tiles, kWPB, 1u << 14)), dim3(kBlock), 0, st, s, ab, cb, sup, own,
+                     total);
   NUC34_LAUNCH_CHECK();
 }
 

@@ -51,8 +51,10 @@ struct alignas(16) PeelWord {
 };
 
 // sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
-// (t holds the K4's lowest three vertices); *total += sum of sup.
-void launch_support_bits(const K4Src& s, int32_t* sup, uint32_t* own, uint64_t* total, hipStream_t st);
+// (t holds the K4's lowest three vertices); *total += sum of sup. ab (WA + 1
+// entries) and cb (WC + 1) are scratch: bits-only copies of the rows.
+void launch_support_bits(const K4Src& s, uint64_t WA, uint64_t WC, uint2* ab, uint32_t* cb, int32_t* sup,
+                         uint32_t* own, uint64_t* total, hipStream_t st);
 // Direct variant: the K4 list of every triangle, three partner triangle ids per
 // entry, at lists[3 * koff[t] ..] (koff = exclusive scan of sup): the K4s whose
 // fourth vertex ranks above t's lowest vertex first, in order of its position
