@@ -501,10 +501,10 @@ void build_peel_order(nuc34_ctx* c, PeelOrder& o) {
   const uint64_t T = c->T;
   o.v = k4src(c);
   o.liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
-  o.liveC = c->alloc.get(c->WC * sizeof(PeelWord) + sizeof(PeelWord), "peel c rows");
+  o.liveC = c->alloc.get(c->WC * sizeof(PlaneWord) + sizeof(PlaneWord), "peel c planes");
   o.icut = c->alloc.get((static_cast<uint64_t>(c->n) + 1) * 4, "in-list cuts");
   o.v.PA = o.liveA.as<PeelWord>();
-  o.v.PC = o.liveC.as<PeelWord>();
+  o.v.PC = o.liveC.as<PlaneWord>();
   o.v.icut = o.icut.as<uint32_t>();
   {
     DevBuf cnt = c->alloc.get(c->WA * 4 + 4, "peel row counts");

@@ -165,12 +165,17 @@ __device__ __forceinline__ K4Word k4_word(const K4Src& s, const TriRows& r, uint
     const uint64_t cb = ca + static_cast<uint64_t>(r.jv) * r.WC - static_cast<uint64_t>(r.jw) * r.WC;
     k.Aa = k.Ab = 0;
     if (LIVE) {
-      const PeelWord la = s.PC[ca], lb = s.PC[cb];
-      k.Ia = RankWord{la.bits, la.rank};
-      k.Ib = RankWord{lb.bits, lb.rank};
+      // the planes first (most down words hold no K4); the ranked words only
+      // where the word has K4 bits
+      const PlaneWord la = s.PC[ca], lb = s.PC[cb];
       k.bits = la.live & lb.live & ~((la.dying | lb.dying) & bits_below(r.cut, (wi - r.W) * 32));
       k.Da = la.dying;
       k.Db = lb.dying;
+      k.Ia = k.Ib = RankWord{0u, 0u};
+      if (k.bits) {
+        k.Ia = s.rows.Cm[ca];
+        k.Ib = s.rows.Cm[cb];
+      }
     } else {
       k.Ia = s.rows.Cm[ca];
       k.Ib = s.rows.Cm[cb];
@@ -188,6 +193,15 @@ __device__ __forceinline__ uint32_t peel_rank(const PeelWord* row, uint32_t j, u
   const uint32_t b = j & 31u;
   *state = ((w.live >> b) & 1u) | (((w.dying >> b) & 1u) << 1);
   return w.rank + static_cast<uint32_t>(__popc(w.bits & ((1u << b) - 1u)));
+}
+
+// Rank of bit position j of the c-role row at word offset `row` (its c-slot);
+// *state gets that bit's live (bit 0) and dying (bit 1) planes.
+__device__ __forceinline__ uint32_t c_peel_rank(const K4Src& s, uint64_t row, uint32_t j, uint32_t* state) {
+  const PlaneWord w = s.PC[row + (j >> 5)];
+  const uint32_t b = j & 31u;
+  *state = ((w.live >> b) & 1u) | (((w.dying >> b) & 1u) << 1);
+  return row_rank(s.rows.Cm + row, j);
 }
 
 // Rank of bit position j of the peel row at word offset `row` (Aout bits, peel
@@ -254,7 +268,7 @@ __device__ __forceinline__ void k4_partners(const K4Src& s, const TriRows& r, ui
     q[1] = s.cSlot[rank_in_word(k.Ia, j)].x;  // T{x,u,w}: c-edge a
     // T{x,v,w}: c-edge c, at x's position in N-(v), the in-list position of
     // (x,v), the a-edge of T{x,u,v}
-    q[2] = s.cSlot[LIVE ? peel_rank(s.PC + r.cc, xuv.y, s2) : row_rank(s.rows.Cm + r.cc, xuv.y)].x;
+    q[2] = s.cSlot[LIVE ? c_peel_rank(s, r.cc, xuv.y, s2) : row_rank(s.rows.Cm + r.cc, xuv.y)].x;
   }
 }
 
@@ -835,15 +849,17 @@ __device__ __forceinline__ void mark_dying(const K4Src& s, uint32_t t) {
   atomic_or_u32(&s.PC[x.w[2]].dying, 1u << x.bit[2]);
 }
 
+__device__ __forceinline__ void clear_plane_bit(uint32_t* live, uint32_t bit) {
+  const uint64_t b = 1ull << bit;
+  atomic_and_u64(reinterpret_cast<uint64_t*>(live), ~(b | (b << 32)));
+}
+
 // t is retired: its bits leave both planes (one 64-bit atomic per word).
 __device__ __forceinline__ void unlink_tri(const K4Src& s, uint32_t t) {
   const TriSlots x = tri_slots(s, t);
-  PeelWord* const words[3] = {s.PA + x.w[0], s.PA + x.w[1], s.PC + x.w[2]};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint64_t bit = 1ull << x.bit[k];
-    atomic_and_u64(reinterpret_cast<uint64_t*>(&words[k]->live), ~(bit | (bit << 32)));
-  }
+  clear_plane_bit(&s.PA[x.w[0]].live, x.bit[0]);
+  clear_plane_bit(&s.PA[x.w[1]].live, x.bit[1]);
+  clear_plane_bit(&s.PC[x.w[2]].live, x.bit[2]);
 }
 
 // State words per thread and pass of a level scan.
@@ -1429,10 +1445,7 @@ __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* _
     s.PA[i] = PeelWord{o, 0u, o | s.rows.Ain[i].bits, 0u};
     cnt[i] = static_cast<uint32_t>(__popc(o));
   }
-  for (uint64_t i = tid; i < WC; i += stride) {
-    const RankWord c = s.rows.Cm[i];
-    s.PC[i] = PeelWord{c.bits, c.rank, c.bits, 0u};
-  }
+  for (uint64_t i = tid; i < WC; i += stride) s.PC[i] = PlaneWord{s.rows.Cm[i].bits, 0u};
   for (uint64_t u = tid; u < n; u += stride)  // in-lists are sorted by source id
     icut[u] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(IE, IO[u], IO[u + 1], static_cast<uint32_t>(u)) - IO[u]);
 }

@@ -22,9 +22,9 @@ struct K4Src {
   const uint2* cSlot;
   uint64_t T;
   // Peel rows for the on-demand peel (null elsewhere), word for word beside
-  // (Aout|Ain) and Cm: each word repeats the Aout (or Cm) bits and running rank
+  // (Aout|Ain) and Cm: each PA word repeats the Aout bits and running rank
   // next to two planes, `live` (the triangles not yet retired) and `dying`
-  // (the current round's frontier). The on-demand peel numbers triangles in
+  // (the current round's frontier); PC holds the c-role rows' two planes. The on-demand peel numbers triangles in
   // peel order, by b-edge and then a-edge, which is the running rank of the
   // Aout bits: a dying triangle's K4s with their fourth vertex above its own
   // three find all three partners at consecutive ids along the rows of its
@@ -37,17 +37,25 @@ struct K4Src {
   // third partner's rank lookup reads the word that holds its planes too, so
   // its state costs no load of its own.
   struct PeelWord* PA;
-  struct PeelWord* PC;
+  struct PlaneWord* PC;  // the c-role rows' planes only: their bits and ranks are Cm's
   // per vertex u: in-neighbours with a lower vertex id than u (a prefix of
   // N-(u), whose triangles precede u's in the peel order); on-demand peel only
   const uint32_t* icut;
 };
 
 struct alignas(16) PeelWord {
-  uint32_t bits;  // Aout (PA) or Cm (PC) bits
-  uint32_t rank;  // their running rank: peel-order ids (PA), c-slots (PC)
-  uint32_t live;  // (Aout|Ain) or Cm bits of the triangles not yet retired
+  uint32_t bits;  // Aout bits
+  uint32_t rank;  // their running rank: peel-order ids
+  uint32_t live;  // (Aout|Ain) bits of the triangles not yet retired
   uint32_t dying; // bits of the current round's frontier
+};
+
+// The planes of a c-role row word (beside Cm's bits and c-slot ranks): most
+// down words hold no K4, so the debit reads these 8 bytes first and the ranked
+// word only where the planes leave K4 bits.
+struct alignas(8) PlaneWord {
+  uint32_t live;   // Cm bits of the triangles not yet retired
+  uint32_t dying;  // bits of the current round's frontier
 };
 
 // sup[t] = K4s on t, own[t] = K4s whose fourth vertex ranks above all of t
@@ -183,7 +191,7 @@ void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s);
 // The on-demand peel's rows and ids. launch_live_rows: PA[i] = {Aout[i], -,
 // Aout[i] | Ain[i].bits, 0} and cnt[i] = popcount(Aout[i]) for i < WA (the
 // caller scans cnt into P and sets the ranks with launch_peel_ranks),
-// PC[i] = {Cm[i], Cm[i].bits, 0} for i < WC, and icut for the n vertices of the
+// PC[i] = {Cm[i].bits, 0} for i < WC, and icut for the n vertices of the
 // in-CSR (IO, IE).
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
                       uint32_t* icut, uint32_t* cnt, hipStream_t s);
