@@ -564,7 +564,11 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
                                                                     peel_rounds_per_launch=6)),
                                ("direct", dict(peel_debit_grid=300, k4_list_grid_cap=17)),
                                ("direct", dict(tri_tile_edges=16)), ("ondemand", dict(tri_tile_edges=32)),
-                               ("ondemand", dict(tri_schedule=1)), ("direct", dict(tri_schedule=0))):
+                               ("ondemand", dict(tri_schedule=1)), ("direct", dict(tri_schedule=0)),
+                               # two debit workgroups: their push regions (1024 slots) overflow into
+                               # the shared list in the rounds with more pushes
+                               ("ondemand", dict(peel_debit_grid=2, peel_tile_cap=64)),
+                               ("direct", dict(peel_debit_grid=2))):
             if shape:
                 ctx.set_tuning(**shape)
             ctx.reset()
