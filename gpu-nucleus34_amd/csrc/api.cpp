@@ -53,8 +53,12 @@ struct nuc34_ctx {
   DevBuf IO, IE, IEe, inpos;        // in-CSR of the oriented graph
   DevBuf rowA, rowC, Aout, Ain, Cm;  // bit rows
   uint64_t WA = 0, WC = 0;           // words of the Aout/Ain rows and of the Cm rows
-  DevBuf offA, triA, triBC;
-  DevBuf cSlot;                      // per c-slot: triangle id, in-list position of its a-edge
+  DevBuf offA;
+  DevBuf AoutR;                      // running rank of the Aout bits per word (peel ids)
+  DevBuf triA, triBC;                // by peel id (b-edge, then a-edge)
+  DevBuf canon;                      // canonical (a, b) index per peel id
+  DevBuf cSlot;                      // per c-slot: peel id, in-list position of its a-edge
+  DevBuf ctriA, ctriBC;              // canonical order, built when first asked for
   DevBuf vbins;                      // degree-binned schedule: vertex classes (during the triangle passes)
   // support
   bool has_sup = false;
@@ -63,7 +67,6 @@ struct nuc34_ctx {
   // direct: per-triangle K4 partner lists
   bool has_k4 = false;
   DevBuf koff, lists;
-  DevBuf pmap, psup;  // the lists' peel order: canonical -> peel id, supports in peel order
   // peel result
   bool has_k = false;
   DevBuf kv;
@@ -310,7 +313,7 @@ void reset_stages(nuc34_ctx* c, int from) {
     c->batch_rounds.clear();
   }
   if (from <= 3) {
-    for (DevBuf* b : {&c->koff, &c->lists, &c->pmap, &c->psup}) b->release();
+    for (DevBuf* b : {&c->koff, &c->lists}) b->release();
     c->has_k4 = false;
   }
   if (from <= 2) {
@@ -320,7 +323,7 @@ void reset_stages(nuc34_ctx* c, int from) {
   }
   if (from <= 1) {
     for (DevBuf* b : {&c->IO, &c->IE, &c->IEe, &c->inpos, &c->rowA, &c->rowC, &c->Aout, &c->Ain, &c->Cm, &c->offA,
-                      &c->triA, &c->triBC, &c->cSlot, &c->vbins})
+                      &c->AoutR, &c->triA, &c->triBC, &c->canon, &c->cSlot, &c->ctriA, &c->ctriBC, &c->vbins})
       b->release();
     c->has_tris = false;
     c->T = 0;
@@ -361,7 +364,7 @@ uint32_t tri_bins_min_degree(const nuc34_ctx* c) { return c->tune.tri_schedule =
 
 BitRows bitrows(nuc34_ctx* c) {
   return BitRows{c->rowA.as<uint64_t>(), c->rowC.as<uint64_t>(), c->Aout.as<uint32_t>(), c->Ain.as<RankWord>(),
-                 c->Cm.as<RankWord>()};
+                 c->Cm.as<RankWord>(),       c->AoutR.as<uint32_t>()};
 }
 
 K4Src k4src(nuc34_ctx* c) {
@@ -422,9 +425,8 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
                         c->inpos.as<uint32_t>(), tri_tile(c), pb, c->s);
     });
   });
-  // canonical order: role counts, ids and ranks, then the records at their ids
+  // ids: role counts and ranks, then the records at their peel ids
   c->st.ms_tri_order = timed(c, [&] {
-    const BitRows rows = bitrows(c);
     c->offA = c->alloc.get((m + 1) * 8, "triangle offsets");
     {
       // one exclusive scan of the set bits over all a-role words gives every
@@ -445,17 +447,30 @@ void run_triangles(nuc34_ctx* c, uint32_t parts) {
       launch_set_ranks(c->Ain.as<RankWord>(), pA.as<uint64_t>(), WA, c->s);
       launch_set_ranks(c->Cm.as<RankWord>(), pC.as<uint64_t>(), WC, c->s);
       launch_gather_u64(pA.as<uint64_t>(), c->rowA.as<uint64_t>(), c->offA.as<uint64_t>(), m + 1, c->s);
+      // the peel ids: the same scan over the b-role words
+      exclusive_scan<BitWord, uint64_t>(reinterpret_cast<const BitWord*>(c->Aout.as<uint32_t>()), pA.as<uint64_t>(),
+                                        WA, scratch.as<uint64_t>(), c->s);
+      if (read_scalar<uint64_t>(c, pA.as<uint64_t>() + WA) != c->T)
+        throw std::logic_error("b-role bits disagree with the triangle count");
+      c->AoutR = c->alloc.get(WA * 4 + 4, "b-role ranks");
+      launch_set_ranks32(c->AoutR.as<uint32_t>(), pA.as<uint64_t>(), WA, c->s);
     }
+    const BitRows rows = bitrows(c);
     c->triA = c->alloc.get(c->T * 4 + 4, "triangle a-edges");
     c->triBC = c->alloc.get(c->T * 8 + 8, "triangle b,c edges");
+    c->canon = c->alloc.get(c->T * 4 + 4, "canonical triangle order");
     c->cSlot = c->alloc.get(c->T * 8 + 8, "triangles by c-slot");
+    TriOut out;
+    out.triA = c->triA.as<uint32_t>();
+    out.triBC = c->triBC.as<uint64_t>();
+    out.canon = c->canon.as<uint32_t>();
+    out.cSlot = c->cSlot.as<uint2>();
     const VertexBins bins{c->vbins.as<uint32_t>(), c->vbins.as<uint32_t>() + 3 * static_cast<uint64_t>(n), n,
                           tri_bins_min_degree(c)};
     ktime(c, K_TRI_EMIT, ranges.size(), [&] {
       for (const auto& r : ranges)
         launch_tri_emit(c->O.as<uint32_t>(), c->E.as<uint32_t>(), c->src.as<uint32_t>(), r.first, r.second, rows,
-                        c->inpos.as<uint32_t>(), c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(),
-                        c->cSlot.as<uint2>(), tri_tile(c), c->vbins ? &bins : nullptr, c->s);
+                        c->inpos.as<uint32_t>(), out, tri_tile(c), c->vbins ? &bins : nullptr, c->s);
     });
     c->vbins.release();
   });
@@ -486,19 +501,16 @@ void run_support(nuc34_ctx* c) {
   c->st.four_cliques = c->Q;
 }
 
-// Peel order: triangles numbered by b-edge, then a-edge (the running ranks of
-// the Aout bits), so that a dying triangle's K4 partners above it sit at
-// consecutive ids along its own rows and the peel's decrements share cache
-// lines. o.v enumerates K4s from the peel rows (live plane over Aout | Ain,
-// dying plane empty) and names partners by peel id; o.map takes canonical
-// ids to peel ids, o.sup holds the supports in peel order.
-struct PeelOrder {
-  DevBuf liveA, liveC, icut, map, triA, triBC, cslot, sup, bad;
+// The peel rows: the Aout bits with their peel ranks beside a live plane (over
+// Aout | Ain) and a dying plane, the c-role rows' planes, and the in-list cuts.
+// o.v enumerates K4s from them (fresh planes: every K4) and names partners by
+// peel id, the order the triangles are numbered in.
+struct PeelRows {
+  DevBuf liveA, liveC, icut;
   K4Src v{};
 };
 
-void build_peel_order(nuc34_ctx* c, PeelOrder& o) {
-  const uint64_t T = c->T;
+void build_peel_rows(nuc34_ctx* c, PeelRows& o) {
   o.v = k4src(c);
   o.liveA = c->alloc.get(c->WA * sizeof(PeelWord) + sizeof(PeelWord), "peel a/b rows");
   o.liveC = c->alloc.get(c->WC * sizeof(PlaneWord) + sizeof(PlaneWord), "peel c planes");
@@ -506,55 +518,38 @@ void build_peel_order(nuc34_ctx* c, PeelOrder& o) {
   o.v.PA = o.liveA.as<PeelWord>();
   o.v.PC = o.liveC.as<PlaneWord>();
   o.v.icut = o.icut.as<uint32_t>();
-  {
-    DevBuf cnt = c->alloc.get(c->WA * 4 + 4, "peel row counts");
-    DevBuf P = c->alloc.get((c->WA + 1) * 8, "peel row ranks");
-    DevBuf scratch = c->alloc.get(scan_scratch_elems(c->WA) * 8, "scan scratch");
-    launch_live_rows(o.v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, o.icut.as<uint32_t>(),
-                     cnt.as<uint32_t>(), c->s);
-    exclusive_scan<uint32_t, uint64_t>(cnt.as<uint32_t>(), P.as<uint64_t>(), c->WA, scratch.as<uint64_t>(), c->s);
-    launch_peel_ranks(o.v.PA, P.as<uint64_t>(), c->WA, c->s);
-  }
-  o.map = c->alloc.get(T * 4, "peel order of the triangles");
-  o.triA = c->alloc.get(T * 4, "triangles in peel order");
-  o.triBC = c->alloc.get(T * 8, "triangles in peel order");
-  o.cslot = c->alloc.get(T * 8 + 8, "c-slots in peel order");
-  o.sup = c->alloc.get(T * 4 + 4, "supports in peel order");
-  o.bad = c->alloc.get(8, "peel order check");
-  NUC34_HIP(hipMemsetAsync(o.bad.as<void>(), 0, 8, c->s));
-  launch_peel_order(o.v, c->sup.as<int32_t>(), o.map.as<uint32_t>(), o.triA.as<uint32_t>(), o.triBC.as<uint64_t>(),
-                    o.sup.as<int32_t>(), o.cslot.as<uint2>(), o.bad.as<uint32_t>(), c->s);
-  o.v.triA = o.triA.as<uint32_t>();
-  o.v.triBC = o.triBC.as<uint64_t>();
-  o.v.cSlot = o.cslot.as<uint2>();
+  launch_live_rows(o.v, c->WA, c->WC, c->IO.as<uint32_t>(), c->IE.as<uint32_t>(), c->n, o.icut.as<uint32_t>(), c->s);
 }
 
-void check_peel_order(nuc34_ctx* c, const PeelOrder& o) {
-  if (read_scalar<uint32_t>(c, o.bad.as<void>())) throw std::logic_error("peel order: a triangle is missing from its b-edge's row");
+// The triangle arrays in canonical (a, b) order, for the callers that copy
+// triangles out (built once per triangle listing).
+void ensure_canonical_tris(nuc34_ctx* c) {
+  if (c->ctriA || !c->T) return;
+  c->ctriA = c->alloc.get(c->T * 4 + 4, "triangles in canonical order");
+  c->ctriBC = c->alloc.get(c->T * 8 + 8, "triangles in canonical order");
+  launch_canon_tris(c->triA.as<uint32_t>(), c->triBC.as<uint64_t>(), c->canon.as<uint32_t>(), c->T,
+                    c->ctriA.as<uint32_t>(), c->ctriBC.as<uint64_t>(), c->s);
 }
 
-// Direct variant: every triangle's K4 list (three partners per K4), in peel
-// order: list offsets from the supports in peel order, partners by peel id.
+// Direct variant: every triangle's K4 list (three partners per K4, by peel id),
+// from the peel rows with fresh planes.
 void run_k4_lists(nuc34_ctx* c) {
   const uint64_t T = c->T, Q = c->Q;
   c->st.ms_k4_list = timed(c, [&] {
-    PeelOrder o;
-    build_peel_order(c, o);
+    PeelRows o;
+    build_peel_rows(c, o);
     c->koff = c->alloc.get((T + 1) * 8, "K4 list offsets");
     {
       DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<int32_t, uint64_t>(o.sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
+      exclusive_scan<int32_t, uint64_t>(c->sup.as<int32_t>(), c->koff.as<uint64_t>(), T, scratch.as<uint64_t>(),
                                         c->s);
     }
-    check_peel_order(c, o);
     if (read_scalar<uint64_t>(c, c->koff.as<uint64_t>() + T) != 4 * Q)
       throw std::logic_error("K4 list sizes disagree with the support sum");
     c->lists = c->alloc.get(4 * Q * 12 + 12, "K4 partner lists");
     ktime(c, K_K4_LISTS, 1, [&] {
       launch_k4_lists(o.v, c->koff.as<uint64_t>(), c->lists.as<uint32_t>(), c->tune.k4_list_grid_cap, true, c->s);
     });
-    c->pmap = std::move(o.map);
-    c->psup = std::move(o.sup);
   });
   c->has_k4 = true;
 }
@@ -619,12 +614,11 @@ void run_peel(nuc34_ctx* c, int variant) {
   DevBuf ctlb = c->alloc.get(2 * sizeof(PeelCtl), "peel control");
   PeelBufs pb{dd.as<int32_t>(),  stw.as<uint32_t>(), c->kv.as<int32_t>(),
               b0.as<uint32_t>(), b1.as<uint32_t>(),  ctlb.as<PeelCtl>(), T};
-  // Both variants peel in peel order (supports, states, frontiers and k by
-  // peel id); k goes back to the canonical order at the end. On demand the
-  // order and its peel rows are built here, fresh planes for every peel; the
-  // direct variant's came with its K4 lists.
+  // Both variants peel by peel id (supports, states, frontiers, k); k goes
+  // back to the canonical order at the end. On demand the peel rows are built
+  // here, fresh planes for every peel.
   K4Src v{};
-  PeelOrder o;
+  PeelRows o;
   DevBuf pkv = c->alloc.get(T * 4 + 4, "k values in peel order");
   pb.kv = pkv.as<int32_t>();
   // A loaded batch whose graphs are all small: one workgroup per graph, the
@@ -695,13 +689,11 @@ void run_peel(nuc34_ctx* c, int variant) {
   PeelCtl hs[2] = {};
   PeelCtl h{};
   c->st.ms_peel_init = timed(c, [&] {
-    if (variant == NUC34_DIRECT) {
-      launch_peel_init(pb, c->psup.as<int32_t>(), c->s);
-      return;
+    if (variant != NUC34_DIRECT) {
+      build_peel_rows(c, o);
+      v = o.v;
     }
-    build_peel_order(c, o);
-    v = o.v;
-    launch_peel_init(pb, o.sup.as<int32_t>(), c->s);
+    launch_peel_init(pb, c->sup.as<int32_t>(), c->s);
   });
   if (by_graph) {
     c->st.ms_peel = c->st.ms_peel_init + timed(c, [&] {
@@ -810,11 +802,9 @@ void run_peel(nuc34_ctx* c, int variant) {
   });
   }
   if (h.error) throw std::logic_error("peel: inconsistent state (code " + std::to_string(h.error) + ")");
-  {
-    const DevBuf& map = variant == NUC34_DIRECT ? c->pmap : o.map;
-    c->st.ms_peel += timed(c, [&] { launch_gather_k(pkv.as<int32_t>(), map.as<uint32_t>(), T, c->kv.as<int32_t>(), c->s); });
-    if (variant != NUC34_DIRECT) check_peel_order(c, o);
-  }
+  c->st.ms_peel += timed(c, [&] {
+    launch_canon_u32(pkv.as<uint32_t>(), c->canon.as<uint32_t>(), T, c->kv.as<uint32_t>(), c->s);
+  });
   c->st.rounds = h.round;
   c->st.k_max = h.level;
   c->st.unpeeled = T - h.peeled;
@@ -1299,12 +1289,13 @@ int nuc34_copy_triangles(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_t*
   if (first > c->T || count > c->T - first) return fail(c, NUC34_E_INVALID, "triangle range out of bounds");
   return guard(c, [&]() -> int {
     if (!count) return NUC34_OK;
+    ensure_canonical_tris(c);
     // formatted as (a, b, c) triples on the device, copied back in chunks
     const uint64_t chunk = std::min<uint64_t>(count, 16ull << 20);
     DevBuf buf = c->alloc.get(chunk * 12, "triangle copy");
     for (uint64_t done = 0; done < count; done += chunk) {
       const uint64_t n = std::min(chunk, count - done);
-      launch_tri_abc(c->triA.as<uint32_t>() + first + done, c->triBC.as<uint64_t>() + first + done, n,
+      launch_tri_abc(c->ctriA.as<uint32_t>() + first + done, c->ctriBC.as<uint64_t>() + first + done, n,
                      buf.as<uint32_t>(), c->s);
       to_host(c, abc + 3 * done, buf.as<void>(), n * 12);  // the buffer is reused only after this returns
     }
@@ -1341,7 +1332,10 @@ int nuc34_support(nuc34_ctx* c, uint64_t* Q) {
 int nuc34_copy_support(nuc34_ctx* c, int32_t* support) {
   if (!c || !c->has_sup) return fail(c, NUC34_E_STATE, "support not computed");
   return guard(c, [&]() -> int {
-    if (c->T) to_host(c, support, c->sup.as<void>(), c->T * 4);
+    if (!c->T) return NUC34_OK;
+    DevBuf cs = c->alloc.get(c->T * 4, "support in canonical order");
+    launch_canon_u32(c->sup.as<uint32_t>(), c->canon.as<uint32_t>(), c->T, cs.as<uint32_t>(), c->s);
+    to_host(c, support, cs.as<void>(), c->T * 4);
     return NUC34_OK;
   });
 }
@@ -1418,11 +1412,19 @@ int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_
   return guard(c, [&]() -> int {
     if (!count) return NUC34_OK;
     const uint64_t T = c->T;
+    // the listing walks the triangles in canonical order, with canonical ids
+    // (the a-role ranks)
+    ensure_canonical_tris(c);
+    K4Src kc = k4src(c);
+    kc.triA = c->ctriA.as<uint32_t>();
+    kc.triBC = c->ctriBC.as<uint64_t>();
+    kc.cSlot = nullptr;
     DevBuf k4off = c->alloc.get((T + 1) * 8, "owned K4 offsets");
     {
+      DevBuf own = c->alloc.get(T * 4, "owned K4 counts in canonical order");
+      launch_canon_u32(c->own.as<uint32_t>(), c->canon.as<uint32_t>(), T, own.as<uint32_t>(), c->s);
       DevBuf scratch = c->alloc.get(scan_scratch_elems(T) * 8, "scan scratch");
-      exclusive_scan<uint32_t, uint64_t>(c->own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(),
-                                         c->s);
+      exclusive_scan<uint32_t, uint64_t>(own.as<uint32_t>(), k4off.as<uint64_t>(), T, scratch.as<uint64_t>(), c->s);
     }
     // only the triangles whose owned K4s reach [first, first + count) are walked
     DevBuf bounds = c->alloc.get(16, "K4 copy bounds");
@@ -1432,9 +1434,9 @@ int nuc34_copy_fourcliques(nuc34_ctx* c, uint64_t first, uint64_t count, uint32_
     NUC34_HIP(hipStreamSynchronize(c->s));
     if (tb[0] > tb[1] || tb[1] >= T) throw std::logic_error("owned K4 offsets inconsistent");
     DevBuf k4 = c->alloc.get(count * 8, "owned K4s");
-    launch_k4_owned(k4src(c), k4off.as<uint64_t>(), tb[0], tb[1] + 1, first, count, k4.as<uint64_t>(), c->s);
+    launch_k4_owned(kc, k4off.as<uint64_t>(), tb[0], tb[1] + 1, first, count, k4.as<uint64_t>(), c->s);
     DevBuf out = c->alloc.get(count * 16, "K4 copy");
-    launch_k4_quads(k4src(c), k4.as<uint64_t>(), 0, count, out.as<uint32_t>(), c->s);
+    launch_k4_quads(kc, k4.as<uint64_t>(), 0, count, out.as<uint32_t>(), c->s);
     NUC34_HIP(hipMemcpyAsync(quds, out.as<void>(), count * 16, hipMemcpyDeviceToHost, c->s));
     NUC34_HIP(hipStreamSynchronize(c->s));
     return NUC34_OK;
@@ -1691,7 +1693,8 @@ extern "C" int nuc34_decompose_noalloc(int device, const uint32_t* O, const uint
     c.st.peak_bytes = c.alloc.peak;
     if (c.T) {
       DevBuf buf = c.alloc.get(c.T * 12, "triangle copy");
-      launch_tri_abc(c.triA.as<uint32_t>(), c.triBC.as<uint64_t>(), c.T, buf.as<uint32_t>(), c.s);
+      launch_tri_abc_canon(c.triA.as<uint32_t>(), c.triBC.as<uint64_t>(), c.canon.as<uint32_t>(), c.T,
+                           buf.as<uint32_t>(), c.s);
       to_host(&c, abc, buf.as<void>(), c.T * 12);
       to_host(&c, k, c.kv.as<void>(), c.T * 4);
     }

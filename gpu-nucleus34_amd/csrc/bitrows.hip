@@ -68,6 +68,12 @@ __global__ void k_set_ranks(RankWord* __restrict__ w, const uint64_t* __restrict
     w[i].rank = static_cast<uint32_t>(P[i]);
 }
 
+__global__ void k_set_ranks32(uint32_t* __restrict__ r, const uint64_t* __restrict__ P, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    r[i] = static_cast<uint32_t>(P[i]);
+}
+
 }  // namespace
 
 void build_in_csr(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint32_t n, uint64_t m, uint32_t* IO,
@@ -112,6 +118,12 @@ void launch_row_words(const uint32_t* O, const uint32_t* IO, const uint32_t* src
 void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStream_t s) {
   if (!m) return;
   hipLaunchKernelGGL(k_edge_triangles, dim3(grid_for(m, 256, 1u << 16)), dim3(256), 0, s, r, m, cnt);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_set_ranks32(uint32_t* r, const uint64_t* P, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_set_ranks32, dim3(grid_for(n, 256, 1u << 16)), dim3(256), 0, s, r, P, n);
   NUC34_LAUNCH_CHECK();
 }
 

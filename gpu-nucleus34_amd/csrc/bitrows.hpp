@@ -34,6 +34,14 @@ struct BitRows {
   uint32_t* Aout;
   RankWord* Ain;
   RankWord* Cm;
+  // running rank of the Aout bits at each Aout word: a triangle's peel id is
+  // the rank of its bit in its b-edge's row (set after the bits pass)
+  const uint32_t* AoutR = nullptr;
+};
+
+// A bare bit word, scanned by its set bits (exclusive_scan over BitWord).
+struct BitWord {
+  uint32_t bits;
 };
 
 // Rank of bit j (j < 32) of a ranked word: its rank plus the set bits below j.
@@ -68,5 +76,7 @@ void launch_edge_triangles(const BitRows& r, uint64_t m, uint32_t* cnt, hipStrea
 // (exclusive_scan over RankWord). Rows lie edge after edge, so P at a row's
 // first word is the edge's first triangle id (Ain) or first c-slot (Cm).
 void launch_set_ranks(RankWord* w, const uint64_t* P, uint64_t n, hipStream_t s);
+// r[i] = P[i] for i < n (ranks below 2^32).
+void launch_set_ranks32(uint32_t* r, const uint64_t* P, uint64_t n, hipStream_t s);
 
 }  // namespace nuc34

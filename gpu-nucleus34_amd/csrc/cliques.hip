@@ -1433,69 +1433,19 @@ __global__ __launch_bounds__(kBlock) void k_residual_sum(const int32_t* __restri
   }
 }
 
-// Peel rows: PA holds the Aout bits (their ranks follow from a scan of cnt),
-// the live plane over Aout | Ain and an empty dying plane; PC the Cm words.
+// Peel rows: PA holds the Aout bits with their peel ranks, the live plane over
+// Aout | Ain and an empty dying plane; PC the Cm bits as the live plane.
 __global__ void k_live_rows(K4Src s, uint64_t WA, uint64_t WC, const uint32_t* __restrict__ IO,
-                            const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut,
-                            uint32_t* __restrict__ cnt) {
+                            const uint32_t* __restrict__ IE, uint32_t n, uint32_t* __restrict__ icut) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (uint64_t i = tid; i < WA; i += stride) {
     const uint32_t o = s.rows.Aout[i];
-    s.PA[i] = PeelWord{o, 0u, o | s.rows.Ain[i].bits, 0u};
-    cnt[i] = static_cast<uint32_t>(__popc(o));
+    s.PA[i] = PeelWord{o, s.rows.AoutR[i], o | s.rows.Ain[i].bits, 0u};
   }
   for (uint64_t i = tid; i < WC; i += stride) s.PC[i] = PlaneWord{s.rows.Cm[i].bits, 0u};
   for (uint64_t u = tid; u < n; u += stride)  // in-lists are sorted by source id
     icut[u] = static_cast<uint32_t>(lower_bound_dev<uint32_t>(IE, IO[u], IO[u + 1], static_cast<uint32_t>(u)) - IO[u]);
-}
-
-__global__ void k_peel_ranks(PeelWord* __restrict__ w, const uint64_t* __restrict__ P, uint64_t n) {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    w[i].rank = static_cast<uint32_t>(P[i]);
-}
-
-// Peel order: triangle i of the canonical order gets id map[i], the rank of
-// its bit in the peel row of its b-edge (at its highest vertex). Its edges and
-// support move to that id. A bit found clear sets *bad (no write).
-__global__ __launch_bounds__(256) void k_peel_order(K4Src s, const int32_t* __restrict__ sup,
-                                                    uint32_t* __restrict__ map, uint32_t* __restrict__ triA2,
-                                                    uint64_t* __restrict__ triBC2, int32_t* __restrict__ sup2,
-                                                    uint32_t* __restrict__ bad) {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < s.T;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t a = s.triA[i];
-    const uint64_t bc = s.triBC[i];
-    const uint32_t b = static_cast<uint32_t>(bc >> 32);
-    const uint32_t jw = a - s.O[s.src[b]];
-    const PeelWord w = s.PA[s.rows.rowA[b] + (jw >> 5)];
-    const uint32_t id = rank_in_word(RankWord{w.bits, w.rank}, jw & 31u);
-    if (!((w.bits >> (jw & 31u)) & 1u) || id >= s.T) {
-      atomic_or_u32(bad, 1u);
-      continue;
-    }
-    map[i] = id;
-    triA2[id] = a;
-    triBC2[id] = bc;
-    sup2[id] = sup[i];
-  }
-}
-
-__global__ void k_cslot_order(const uint2* __restrict__ cs, const uint32_t* __restrict__ map, uint64_t T,
-                              uint2* __restrict__ out) {
-  for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < T;
-       r += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint2 e = cs[r];
-    out[r] = make_uint2(map[e.x], e.y);
-  }
-}
-
-__global__ void k_gather_k(const int32_t* __restrict__ kp, const uint32_t* __restrict__ map, uint64_t T,
-                           int32_t* __restrict__ kv) {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-    kv[i] = kp[map[i]];
 }
 
 }  // namespace
@@ -1550,32 +1500,10 @@ void launch_k4_quads(const K4Src& s, const uint64_t* k4owned, uint64_t first, ui
 }
 
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
-                      uint32_t* icut, uint32_t* cnt, hipStream_t s) {
+                      uint32_t* icut, hipStream_t s) {
   const uint64_t most = WA > WC ? (WA > n ? WA : n) : (WC > n ? WC : n);
   if (!most) return;
-  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC, IO, IE, n, icut,
-                     cnt);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_peel_ranks(PeelWord* PA, const uint64_t* P, uint64_t WA, hipStream_t s) {
-  if (!WA) return;
-  hipLaunchKernelGGL(k_peel_ranks, dim3(grid_for(WA, 256, 1u << 16)), dim3(256), 0, s, PA, P, WA);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_peel_order(const K4Src& src, const int32_t* sup, uint32_t* map, uint32_t* triA2, uint64_t* triBC2,
-                       int32_t* sup2, uint2* cslot2, uint32_t* bad, hipStream_t s) {
-  if (!src.T) return;
-  const unsigned grid = grid_for(src.T, 256, 1u << 16);
-  hipLaunchKernelGGL(k_peel_order, dim3(grid), dim3(256), 0, s, src, sup, map, triA2, triBC2, sup2, bad);
-  hipLaunchKernelGGL(k_cslot_order, dim3(grid), dim3(256), 0, s, src.cSlot, map, src.T, cslot2);
-  NUC34_LAUNCH_CHECK();
-}
-
-void launch_gather_k(const int32_t* kp, const uint32_t* map, uint64_t T, int32_t* kv, hipStream_t s) {
-  if (!T) return;
-  hipLaunchKernelGGL(k_gather_k, dim3(grid_for(T, 256, 1u << 16)), dim3(256), 0, s, kp, map, T, kv);
+  hipLaunchKernelGGL(k_live_rows, dim3(grid_for(most, 256, 1u << 14)), dim3(256), 0, s, src, WA, WC, IO, IE, n, icut);
   NUC34_LAUNCH_CHECK();
 }
 

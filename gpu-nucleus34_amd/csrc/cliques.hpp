@@ -28,8 +28,8 @@ struct K4Src {
   // peel order, by b-edge and then a-edge, which is the running rank of the
   // Aout bits: a dying triangle's K4s with their fourth vertex above its own
   // three find all three partners at consecutive ids along the rows of its
-  // own edges, so their decrements share cache lines. Its K4Src then holds
-  // triA, triBC and cSlot in that order. A triangle's three bits of both planes are
+  // own edges, so their decrements share cache lines. triA, triBC and cSlot
+  // are in that order everywhere. A triangle's three bits of both planes are
   // cleared after the round it dies in, so a dying triangle enumerates only
   // K4s whose two partners on its lowest vertex's rows are still alive or dying
   // in the same round, and it drops at once, before any partner lookup, every
@@ -188,20 +188,10 @@ void launch_peel_graphs(const PeelBufs& p, const K4Src& src, const uint64_t* kof
 // Sum of the residual supports after the peel into *total (zeroed by the
 // caller): the peel applied 4Q minus this many support decrements.
 void launch_residual_sum(const PeelBufs& p, int64_t* total, hipStream_t s);
-// The on-demand peel's rows and ids. launch_live_rows: PA[i] = {Aout[i], -,
-// Aout[i] | Ain[i].bits, 0} and cnt[i] = popcount(Aout[i]) for i < WA (the
-// caller scans cnt into P and sets the ranks with launch_peel_ranks),
-// PC[i] = {Cm[i].bits, 0} for i < WC, and icut for the n vertices of the
-// in-CSR (IO, IE).
+// The on-demand peel's rows: PA[i] = {Aout[i], AoutR[i], Aout[i] | Ain[i].bits,
+// 0} for i < WA, PC[i] = {Cm[i].bits, 0} for i < WC, and icut for the n
+// vertices of the in-CSR (IO, IE).
 void launch_live_rows(const K4Src& src, uint64_t WA, uint64_t WC, const uint32_t* IO, const uint32_t* IE, uint32_t n,
-                      uint32_t* icut, uint32_t* cnt, hipStream_t s);
-void launch_peel_ranks(PeelWord* PA, const uint64_t* P, uint64_t WA, hipStream_t s);
-// Peel order (src with PA ranked): map[i] = peel id of canonical triangle i;
-// triA2, triBC2, sup2 and cslot2 are triA, triBC, sup and cSlot in peel order.
-// *bad gets 1 if a triangle's bit is missing from its b-edge's row.
-void launch_peel_order(const K4Src& src, const int32_t* sup, uint32_t* map, uint32_t* triA2, uint64_t* triBC2,
-                       int32_t* sup2, uint2* cslot2, uint32_t* bad, hipStream_t s);
-// kv[i] = kp[map[i]] for i < T: k back in canonical order.
-void launch_gather_k(const int32_t* kp, const uint32_t* map, uint64_t T, int32_t* kv, hipStream_t s);
+                      uint32_t* icut, hipStream_t s);
 
 }  // namespace nuc34

@@ -12,6 +12,7 @@ __device__ __forceinline__ T scan_value(T x) {
   return x;
 }
 __device__ __forceinline__ uint32_t scan_value(RankWord w) { return static_cast<uint32_t>(__popc(w.bits)); }
+__device__ __forceinline__ uint32_t scan_value(BitWord w) { return static_cast<uint32_t>(__popc(w.bits)); }
 
 // Block-wide inclusive scan of one value per thread; returns this thread's
 // inclusive prefix and writes the block total to *total.
@@ -119,5 +120,6 @@ template void exclusive_scan<uint32_t, uint64_t>(const uint32_t*, uint64_t*, uin
 template void exclusive_scan<int32_t, uint64_t>(const int32_t*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template void exclusive_scan<uint64_t, uint64_t>(const uint64_t*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 template void exclusive_scan<RankWord, uint64_t>(const RankWord*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
+template void exclusive_scan<BitWord, uint64_t>(const BitWord*, uint64_t*, uint64_t, uint64_t*, hipStream_t);
 
 }  // namespace nuc34

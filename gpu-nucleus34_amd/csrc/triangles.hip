@@ -18,9 +18,13 @@
 //              a-role row of a (offA[a] plus the a-role bits below it), i.e.
 //              exactly its place in the reference's (a, b)-sorted triangle list
 //              (tri_t::operator<, utils.cuh:93-102), with no sort and no
-//              atomics; its id and its a-edge's in-list position are also
-//              stored at its c-slot (cSlot), the rank of the lowest vertex's
-//              in-list position in the c-role row of c.
+//              atomics. The triangle is written at its peel id instead, the
+//              rank of w's position in the b-role row of b (by b-edge, then
+//              a-edge: the order the peel's decrements share cache lines in,
+//              and the order this pass enumerates in, so the writes coalesce),
+//              with its canonical index beside it; its peel id and its a-edge's
+//              in-list position are also stored at its c-slot (cSlot), the rank
+//              of the lowest vertex's in-list position in the c-role row of c.
 //
 // Degree-binned schedule (tri_schedule 1, or 2 from out-degree 9): the edges
 // of an owner u with out-degree d <= 640 are enumerated a vertex at a time
@@ -52,13 +56,23 @@ __device__ __forceinline__ uint32_t search_u32(P list, uint32_t lo, uint32_t hi,
   return lo;
 }
 
+// Emit pass: triangle (a, b, c) with w at position jw of N+(u) (b = (u, v))
+// goes to its peel id, the rank of its bit in b's b-role row (word wq of the
+// rows), with its canonical index p and its c-slot.
+__device__ __forceinline__ void emit_tri(const BitRows& rows, const TriOut& out, uint32_t a, uint32_t b, uint32_t c,
+                                         uint32_t jw, uint64_t wq, uint32_t p, const uint32_t* __restrict__ inpos) {
+  const uint32_t q = rows.AoutR[wq] + static_cast<uint32_t>(__popc(rows.Aout[wq] & ((1u << (jw & 31u)) - 1u)));
+  out.triA[q] = a;
+  out.triBC[q] = (static_cast<uint64_t>(b) << 32) | c;
+  out.canon[q] = p;
+  out.cSlot[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = uint2{q, inpos[a]};
+}
+
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
                                                      const uint32_t* __restrict__ src, uint64_t e_begin,
                                                      uint64_t e_end, BitRows rows,
-                                                     const uint32_t* __restrict__ inpos,
-                                                     uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                                     uint2* __restrict__ cSlot,
+                                                     const uint32_t* __restrict__ inpos, TriOut out,
                                                      uint32_t G, uint32_t skip_lo, uint32_t skip_hi) {
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave + 1];
   __shared__ uint32_t s_l0[kWavesPerBlock][kWave];
@@ -138,9 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_enum(const uint32_t* __restrict_
             atomicOr(&rows.Cm[rows.rowC[c] + (iu >> 5)].bits, 1u << (iu & 31));
           } else {
             const uint32_t p = row_rank(rows.Ain + rows.rowA[a], jv);
-            triA[p] = a;
-            triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
-            cSlot[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = uint2{p, inpos[a]};
+            emit_tri(rows, out, a, b, c, jw, rows.rowA[b] + (jw >> 5), p, inpos);
           }
         }
       }
@@ -198,9 +210,8 @@ struct VertexLds {
 // of a group (tid = thread index in the group).
 template <bool EMIT, int NT, int D>
 __device__ __forceinline__ void tri_vertex(uint32_t u, const uint32_t* __restrict__ O, const uint32_t* __restrict__ E,
-                                           const BitRows& rows, const uint32_t* __restrict__ inpos,
-                                           uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                           uint2* __restrict__ cSlot, VertexLds<NT, D>& L, unsigned tid) {
+                                           const BitRows& rows, const uint32_t* __restrict__ inpos, const TriOut& out,
+                                           VertexLds<NT, D>& L, unsigned tid) {
   const uint32_t ob = O[u];
   const uint32_t d = O[u + 1] - ob;
   const uint32_t W = (d + 31) >> 5;
@@ -249,9 +260,7 @@ __device__ __forceinline__ void tri_vertex(uint32_t u, const uint32_t* __restric
     } else {
       const RankWord* a_rows = reinterpret_cast<const RankWord*>(L.rows);
       const uint32_t p = row_rank(a_rows + jw * W, j);
-      triA[p] = a;
-      triBC[p] = (static_cast<uint64_t>(b) << 32) | c;
-      cSlot[row_rank(rows.Cm + rows.rowC[c], inpos[b])] = uint2{p, inpos[a]};
+      emit_tri(rows, out, a, b, c, jw, r0 + static_cast<uint64_t>(j) * W + (jw >> 5), p, inpos);
     }
   }
   group_sync<NT>();
@@ -292,9 +301,7 @@ __global__ void k_vertex_bins(const uint32_t* __restrict__ O, uint32_t n, uint32
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_tri_vertex_wave(const uint32_t* __restrict__ O,
                                                             const uint32_t* __restrict__ E, BitRows rows,
-                                                            const uint32_t* __restrict__ inpos,
-                                                            uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                                            uint2* __restrict__ cSlot,
+                                                            const uint32_t* __restrict__ inpos, TriOut out,
                                                             const uint32_t* __restrict__ list,
                                                             const uint32_t* __restrict__ count, uint64_t e_begin,
                                                             uint64_t e_end) {
@@ -305,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_tri_vertex_wave(const uint32_t* __re
        i += static_cast<uint64_t>(gridDim.x) * kWavesPerBlock) {
     const uint32_t u = list[i];
     if (O[u] < e_begin || O[u] >= e_end) continue;  // another segment's vertex
-    tri_vertex<EMIT, kWave, kVertexD0>(u, O, E, rows, inpos, triA, triBC, cSlot, lds[w], lane_id());
+    tri_vertex<EMIT, kWave, kVertexD0>(u, O, E, rows, inpos, out, lds[w], lane_id());
   }
 }
 
@@ -313,9 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_vertex_wave(const uint32_t* __re
 template <bool EMIT, int NT, int D>
 __global__ __launch_bounds__(NT) void k_tri_vertex_group(const uint32_t* __restrict__ O,
                                                          const uint32_t* __restrict__ E, BitRows rows,
-                                                         const uint32_t* __restrict__ inpos,
-                                                         uint32_t* __restrict__ triA, uint64_t* __restrict__ triBC,
-                                                         uint2* __restrict__ cSlot, const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ inpos, TriOut out,
+                                                         const uint32_t* __restrict__ list,
                                                          const uint32_t* __restrict__ count, uint64_t e_begin,
                                                          uint64_t e_end) {
   __shared__ VertexLds<NT, D> lds;
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_tri_vertex_group(const uint32_t* __restr
   for (uint64_t i = blockIdx.x; i < nv; i += gridDim.x) {
     const uint32_t u = list[i];
     if (O[u] < e_begin || O[u] >= e_end) continue;
-    tri_vertex<EMIT, NT, D>(u, O, E, rows, inpos, triA, triBC, cSlot, lds, threadIdx.x);
+    tri_vertex<EMIT, NT, D>(u, O, E, rows, inpos, out, lds, threadIdx.x);
   }
 }
 
@@ -335,6 +341,36 @@ __global__ void k_tri_abc(const uint32_t* __restrict__ triA, const uint64_t* __r
     abc[3 * i] = triA[i];
     abc[3 * i + 1] = static_cast<uint32_t>(bc >> 32);
     abc[3 * i + 2] = static_cast<uint32_t>(bc);
+  }
+}
+
+__global__ void k_canon_tris(const uint32_t* __restrict__ triA, const uint64_t* __restrict__ triBC,
+                             const uint32_t* __restrict__ canon, uint64_t T, uint32_t* __restrict__ ctriA,
+                             uint64_t* __restrict__ ctriBC) {
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < T;
+       q += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t p = canon[q];
+    ctriA[p] = triA[q];
+    ctriBC[p] = triBC[q];
+  }
+}
+
+__global__ void k_canon_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ canon, uint64_t T,
+                            uint32_t* __restrict__ dst) {
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < T;
+       q += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[canon[q]] = src[q];
+}
+
+__global__ void k_tri_abc_canon(const uint32_t* __restrict__ triA, const uint64_t* __restrict__ triBC,
+                                const uint32_t* __restrict__ canon, uint64_t T, uint32_t* __restrict__ abc) {
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < T;
+       q += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t p = canon[q];
+    const uint64_t bc = triBC[q];
+    abc[3 * p] = triA[q];
+    abc[3 * p + 1] = static_cast<uint32_t>(bc >> 32);
+    abc[3 * p + 2] = static_cast<uint32_t>(bc);
   }
 }
 
@@ -363,22 +399,22 @@ namespace {
 
 template <bool EMIT>
 void launch_tri(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint2* cSlot,
-                uint32_t tile, const VertexBins* bins, hipStream_t s) {
+                const BitRows& rows, const uint32_t* inpos, const TriOut& out, uint32_t tile,
+                const VertexBins* bins, hipStream_t s) {
   if (e_end <= e_begin) return;
   const uint32_t G = tile_edges(tile);
   // with bins, the tiles keep only the owners outside the vertex classes
   const uint32_t skip_lo = bins ? bins->min_deg : 0u, skip_hi = bins ? kVertexD2 : 0u;
   hipLaunchKernelGGL(k_tri_enum<EMIT>, dim3(tri_grid(e_end - e_begin, G)), dim3(kBlock), 0, s, O, E, src, e_begin,
-                     e_end, rows, inpos, triA, triBC, cSlot, G, skip_lo, skip_hi);
+                     e_end, rows, inpos, out, G, skip_lo, skip_hi);
   if (bins) {
     const uint64_t n = bins->n;
     hipLaunchKernelGGL(k_tri_vertex_wave<EMIT>, dim3(grid_for(n, kWavesPerBlock, 8192)), dim3(kBlock), 0, s, O, E,
-                       rows, inpos, triA, triBC, cSlot, bins->lists, bins->counts, e_begin, e_end);
+                       rows, inpos, out, bins->lists, bins->counts, e_begin, e_end);
     hipLaunchKernelGGL((k_tri_vertex_group<EMIT, 256, kVertexD1>), dim3(grid_for(n, 1, 4096)), dim3(256), 0, s, O, E,
-                       rows, inpos, triA, triBC, cSlot, bins->lists + n, bins->counts + 1, e_begin, e_end);
+                       rows, inpos, out, bins->lists + n, bins->counts + 1, e_begin, e_end);
     hipLaunchKernelGGL((k_tri_vertex_group<EMIT, 1024, kVertexD2>), dim3(grid_for(n, 1, 512)), dim3(1024), 0, s, O,
-                       E, rows, inpos, triA, triBC, cSlot, bins->lists + 2 * n, bins->counts + 2, e_begin, e_end);
+                       E, rows, inpos, out, bins->lists + 2 * n, bins->counts + 2, e_begin, e_end);
   }
   NUC34_LAUNCH_CHECK();
 }
@@ -388,13 +424,34 @@ void launch_tri(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint6
 void launch_tri_bits(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
                      const BitRows& rows, const uint32_t* inpos, uint32_t tile, const VertexBins* bins,
                      hipStream_t s) {
-  launch_tri<false>(O, E, src, e_begin, e_end, rows, inpos, nullptr, nullptr, nullptr, tile, bins, s);
+  launch_tri<false>(O, E, src, e_begin, e_end, rows, inpos, TriOut{}, tile, bins, s);
 }
 
 void launch_tri_emit(const uint32_t* O, const uint32_t* E, const uint32_t* src, uint64_t e_begin, uint64_t e_end,
-                     const BitRows& rows, const uint32_t* inpos, uint32_t* triA, uint64_t* triBC, uint2* cSlot,
-                     uint32_t tile, const VertexBins* bins, hipStream_t s) {
-  launch_tri<true>(O, E, src, e_begin, e_end, rows, inpos, triA, triBC, cSlot, tile, bins, s);
+                     const BitRows& rows, const uint32_t* inpos, const TriOut& out, uint32_t tile,
+                     const VertexBins* bins, hipStream_t s) {
+  launch_tri<true>(O, E, src, e_begin, e_end, rows, inpos, out, tile, bins, s);
+}
+
+void launch_canon_tris(const uint32_t* triA, const uint64_t* triBC, const uint32_t* canon, uint64_t T,
+                       uint32_t* ctriA, uint64_t* ctriBC, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_canon_tris, dim3(grid_for(T, 256, 1u << 16)), dim3(256), 0, s, triA, triBC, canon, T, ctriA,
+                     ctriBC);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_canon_u32(const uint32_t* src, const uint32_t* canon, uint64_t T, uint32_t* dst, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_canon_u32, dim3(grid_for(T, 256, 1u << 16)), dim3(256), 0, s, src, canon, T, dst);
+  NUC34_LAUNCH_CHECK();
+}
+
+void launch_tri_abc_canon(const uint32_t* triA, const uint64_t* triBC, const uint32_t* canon, uint64_t T,
+                          uint32_t* abc, hipStream_t s) {
+  if (!T) return;
+  hipLaunchKernelGGL(k_tri_abc_canon, dim3(grid_for(T, 256, 1u << 16)), dim3(256), 0, s, triA, triBC, canon, T, abc);
+  NUC34_LAUNCH_CHECK();
 }
 
 void launch_tri_abc(const uint32_t* triA, const uint64_t* triBC, uint64_t count, uint32_t* abc, hipStream_t s) {
