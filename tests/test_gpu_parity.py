@@ -582,6 +582,41 @@ def test_autotune_and_launch_shapes_keep_results(tmp_path):
             ctx.set_tuning(tri_schedule=3)
 
 
+def _clique_book(fans):
+    """A triangle a, b, c and `fans` vertices adjacent to all three, in groups
+    of 1, 2, 3, 4, 5, ... (cycling) that are cliques: each group of g with
+    a, b, c is a K(g+3), so triangle abc has support `fans` and the other
+    triangles k = g."""
+    a, b, c = fans, fans + 1, fans + 2
+    src, dst = [a, a, b], [b, c, c]
+    f, g = 0, 1
+    while f < fans:
+        grp = list(range(f, min(f + g, fans)))
+        for i, x in enumerate(grp):
+            src += [x, x, x] + [x] * (len(grp) - i - 1)
+            dst += [a, b, c] + grp[i + 1:]
+        f += len(grp)
+        g = g % 5 + 1
+    return nuc34.graph_from_pairs(np.array(src, dtype=np.uint32), np.array(dst, dtype=np.uint32))
+
+
+def test_hub_triangle_with_support_above_16_bits():
+    """One triangle in 65536 K4s (its three vertices adjacent to every other
+    vertex, so its down rows span 2049 words) beside many small cliques: both
+    variants equal the oracle's triangles, k and rounds."""
+    g = _clique_book(65536)
+    ref = oracle34.OracleGraph.from_csr(g.O, g.E, orient=True).run("ondemand", 4)
+    assert int(ref["support"].max()) == 65536
+    for variant in ("ondemand", "direct"):
+        print(f"[hub-triangle] {variant}", flush=True)
+        with nuc34.Context() as ctx:
+            ctx.load_graph(g)
+            st = ctx.decompose(variant)
+            assert np.array_equal(ctx.copy_triangles(), ref["tris"]) and np.array_equal(ctx.copy_k(), ref["k"])
+            assert np.array_equal(ctx.copy_support(), ref["support"]), variant
+            assert st["rounds"] == ref["rounds"] and st["k_max"] == ref["K_max"] and st["unpeeled"] == 0
+
+
 def level_peel_sim(tris, O, E):
     """Synchronous level peel of the triangle/K4 hypergraph, brute force: each
     round removes every live triangle whose live support is at most the level,
