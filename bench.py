@@ -61,7 +61,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "gpu-nucleus34_amd")
 GEN = os.path.join(PKG, "_build", "nuc34_gen")
 HASHES = os.path.join(ROOT, "tests", "golden", "hashes.json")
-C3_CPU = os.path.join(ROOT, "profiles", "rmat20k30_cpu_ondemand.json")
+RESTATEMENT_RECORD = os.path.join(ROOT, "profiles", "rmat20k30_cpu_ondemand.json")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
@@ -450,11 +450,12 @@ def peel_atomics(st: dict, rates: dict | None, traffic: dict | None) -> dict | N
             "floor_ms": n / (peak * 1e9) * 1e3 if peak else None}
 
 
-def c3_cpu_record() -> dict | None:
+def restatement_record() -> dict | None:
     """The C++ restatement of the reference's on-demand algorithm timed once on
-    the default workload on the GPU box's host cores (profiles/), if present."""
+    the default workload (R-MAT s20 + 64 K30) on the host cores of the MI355X
+    machine (profiles/), if present."""
     try:
-        return json.load(open(C3_CPU))
+        return json.load(open(RESTATEMENT_RECORD))
     except (OSError, ValueError):
         return None
 
@@ -568,10 +569,10 @@ def main():
         cpu = None
         if not args.no_cpu_baseline:
             cpu = run_same_input_pair(nuc34, tmp, args.cpu_threads)
-            c3 = c3_cpu_record()
-            if c3 and args.workload == "rmat20k30":
-                c3["speedup_vs_this_run"] = c3["decomp_s"] / ((t1 - t0) / args.steps)
-                cpu["full_workload_restatement"] = c3
+            rec = restatement_record()
+            if rec and args.workload == "rmat20k30":
+                rec["speedup_vs_this_run"] = rec["decomp_s"] / ((t1 - t0) / args.steps)
+                cpu["full_workload_restatement"] = rec
         kern = {}
         for s_ in steps:
             for name, k in s_["kernels"].items():

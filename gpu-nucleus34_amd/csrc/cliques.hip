@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) void k_peel_debit_lists(PeelBufs p, PeelCtl
 // On-demand debit: a wave takes tile_width() dying triangles; their K4s are
 // enumerated from the live bit rows, flattened over the lanes. Its waves wait
 // on memory 84% of their cycles (R-MAT s20 + 64 K30), so it runs six per SIMD
-// (80 VGPRs, no spills): C3 1357 -> 1323 ms; seven spill and are slower.
+// (80 VGPRs, no spills): 1357 -> 1323 ms there; seven spill and are slower.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_peel_debit_bits(PeelBufs p, PeelCtl* in, PeelCtl* out, K4Src s,
                                                             uint32_t tile_cap) {
   __shared__ TileLds lds[kWPB];
